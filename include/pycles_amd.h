@@ -1,0 +1,182 @@
+/*
+ * pycles_amd.h -- C ABI of the MI355X (gfx950) dynamical core of PyCLES.
+ *
+ * Plain C: pointers, sizes and a hipStream_t passed as void*.  All field
+ * pointers are DEVICE pointers.  Every entry point returns 0 on success or a
+ * negative PYCLES_ERR_* code; pycles_last_error() gives the message.  No entry
+ * point allocates device memory except the *_create functions, which run once
+ * at set-up (the reference allocates a scratch flux per momentum call,
+ * momentum_advection_weno.h:119/243; nothing here does).
+ *
+ * Device layout of one field ("device layout"): [k][j][i], x fastest, over
+ * nzg = n[2] + 2*gw levels and the n[0]*n[1] interior columns.  x and y are
+ * periodic by index wrap (the reference keeps gw ghost columns there, filled
+ * by a self Sendrecv on one rank, PrognosticVariables.pyx:183-233); the gw
+ * ghost levels in z are kept and filled as the reference's set_bcs does
+ * (prognostic_variables.h:205-270).  pycles_layout_ingress/egress convert to
+ * and from the reference's ghosted [i][j][k] array.
+ *
+ * Precision tags: PYCLES_F64 -- fields are double and all arithmetic is
+ * FP64, reproducing the reference's operation order; PYCLES_F32 -- fields
+ * are float, bulk arithmetic FP32 (profiles and reductions stay FP64).
+ *
+ * Reference profile arrays (rho0, rho0_half, alpha0, alpha0_half, p0,
+ * p0_half) are FP64 device arrays of length nzg, exactly the reference's
+ * ReferenceState arrays (ReferenceState.pyx:20-25).
+ */
+#ifndef PYCLES_AMD_H
+#define PYCLES_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYCLES_OK 0
+#define PYCLES_ERR_ARG -1
+#define PYCLES_ERR_HIP -2
+#define PYCLES_ERR_UNSUPPORTED -3
+#define PYCLES_ERR_NONFINITE -4
+
+#define PYCLES_F64 0
+#define PYCLES_F32 1
+
+/* Latent heat models replacing the reference's C function pointers
+ * double (*L_fp)(double, double) (Microphysics.pxd:65-75). */
+#define PYCLES_LH_VARIABLE 0
+#define PYCLES_LH_CONSTANT 1
+
+/* Mirror of the reference struct DimStruct (Csrc/grid.h:2-24) for one GPU
+ * owning the whole domain: n = interior points, gw = ghost width. */
+typedef struct pycles_dims {
+    int64_t n[3];
+    int64_t gw;
+    double dx[3];
+    double dxi[3];
+} pycles_dims;
+
+/* Reference-state profiles (FP64 device arrays of length nzg). */
+typedef struct pycles_ref {
+    const double *rho0, *rho0_half, *alpha0, *alpha0_half, *p0, *p0_half;
+} pycles_ref;
+
+/* FP32 deviation storage of the scalars.  In PYCLES_F32 the prognostic
+ * scalars are stored as phi' = phi - base(k) with base an FP64 profile of
+ * length nzg (mirrored in the ghost levels); NULL base = stored as is.
+ * Ignored in PYCLES_F64. */
+typedef struct pycles_scalar_base {
+    const double *base[2];
+} pycles_scalar_base;
+
+const char *pycles_last_error(void);
+const char *pycles_version(void);
+int pycles_device_count(void);
+
+/* ---- advection --------------------------------------------------------- */
+/* Replaces ScalarAdvection.update (ScalarAdvection.pyx:69-123) and
+ * MomentumAdvection.update (MomentumAdvection.pyx:41-69), i.e.
+ * compute_advective_fluxes_a + scalar_flux_divergence
+ * (scalar_advection.h:689-727, flux_divergence.h:4-32) and
+ * compute_advective_tendencies_m (momentum_advection.h:13-93).
+ * vel[3] = u, v, w; tend_vel[3] = their tendencies (NULL = no momentum
+ * advection); scal/tend_scal hold nscal (0, 1 or 2) scalars.
+ * Tendencies are accumulated (tend -= div(flux)), per variable in the
+ * reference order x, y, z.  Orders 5, 7, 9 (WENO5/7/9 with centred
+ * interpolation of order 4/6/8 for the advecting velocity); gw must be at
+ * least (order+1)/2. */
+int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                     void *const tend_vel[3], int nscal, const void *const *scal,
+                     void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
+                     int momentum_order, int precision, void *stream);
+
+/* ---- thermodynamics ---------------------------------------------------- */
+/* pv*(T) lookup table (reference Csrc/lookup.h:4-54). */
+typedef struct pycles_lookup {
+    const double *y; /* device, n entries */
+    int64_t n;
+    double x_min, dx, dxi;
+} pycles_lookup;
+
+/* Saturation-adjustment counters (device int64[4]): saturated cells,
+ * secant iterations, cells that hit the iteration cap, cells that fell back
+ * from the learned guess to the secant.  May be NULL. */
+typedef struct pycles_sa_options {
+    int do_qt_clipping;  /* ThermodynamicsSA.pyx:60-67 */
+    int max_iter;        /* secant cap (the reference loop is uncapped) */
+    int latent_heat;     /* PYCLES_LH_* */
+    const void *mlp;     /* learned first guess (pycles_mlp handle) or NULL */
+    int newton_steps;    /* Newton corrections after the learned guess */
+    long long *counters; /* device int64[4] or NULL */
+} pycles_sa_options;
+
+/* Replaces ThermodynamicsSA.update (ThermodynamicsSA.pyx:176-213):
+ * clip_qt, eos_update, buoyancy_update_sa, bvf_sa, thetali_update
+ * (thermodynamics_sa.h:68-220).  s, qt are PV fields (qt clipped in place),
+ * wt the w tendency (buoyancy added); the rest are DV outputs (any may be
+ * NULL except T, qv, ql, alpha). */
+int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt,
+                     void *s, void *qt, void *wt, void *T, void *qv, void *ql, void *alpha,
+                     void *buoyancy, void *theta_rho, void *bvf, void *thetali,
+                     const pycles_scalar_base *base, const pycles_sa_options *opt, int precision,
+                     void *stream);
+
+/* Replaces ThermodynamicsDry.update (ThermodynamicsDry.pyx:105-120). */
+int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref, void *s, void *wt, void *T,
+                      void *alpha, void *buoyancy, void *theta, void *bvf,
+                      const pycles_scalar_base *base, int precision, void *stream);
+
+/* ---- time stepping and boundaries -------------------------------------- */
+/* TimeStepping.update_third (TimeStepping.pyx:131-151) for nv fields,
+ * followed by the z boundary fill of PrognosticVariables.update_all_bcs
+ * (bc_type[v] = 1.0 sym, -1.0 asym).  Tendencies are zeroed. */
+int pycles_rk3_stage(const pycles_dims *dims, int nv, void *const *values, void *const *tend,
+                     void *const *copies, const double *bc_type, int rk_step, double dt,
+                     int precision, void *stream);
+/* z boundary fill only (set_bcs, prognostic_variables.h:205-270). */
+int pycles_apply_z_bcs(const pycles_dims *dims, int nv, void *const *values,
+                       const double *bc_type, int precision, void *stream);
+/* HorizontalMean (ParallelMPI.pyx:196-241) of one field into an FP64
+ * device array of nzg entries (every level, ghosts included). */
+int pycles_level_mean(const pycles_dims *dims, const void *field, double *mean, int precision,
+                      void *stream);
+/* compute_cfl_max (TimeStepping.pyx:222-263) into one FP64 device value
+ * (max over interior of dt*(|u|/dx+|v|/dy+|w|/dz), plus 1e-11). */
+int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
+                   double dt, double *out, int precision, void *stream);
+
+/* ---- pressure projection ----------------------------------------------- */
+typedef struct pycles_pressure_plan pycles_pressure_plan;
+/* Set-up: modified wavenumbers, tridiagonal coefficients and the spectral
+ * workspace (PressureFFTParallel.initialize, PressureFFTParallel.pyx:27-148). */
+int pycles_pressure_plan_create(const pycles_dims *dims, const double *rho0_host,
+                                const double *rho0_half_host, int precision,
+                                pycles_pressure_plan **plan);
+void pycles_pressure_plan_destroy(pycles_pressure_plan *plan);
+/* Replaces PressureSolver.update (PressureSolver.pyx:38-81): remove the
+ * level mean of w, divergence of rho0*u, horizontal real FFT, per-wavenumber
+ * tridiagonal solve in z with the (0,0) mode set to zero, inverse FFT,
+ * pressure z boundary fill, u -= grad p, boundary fill of u, v, w.
+ * p (dynamic pressure) and div (may be NULL) use the field precision.
+ * If cfl_out is non-NULL the CFL maximum of the corrected velocities
+ * (as pycles_cfl_max with dt) is written there too. */
+int pycles_pressure_update(pycles_pressure_plan *plan, const pycles_ref *ref, void *u, void *v,
+                           void *w, void *p, void *div, double dt, double *cfl_out, void *stream);
+/* second_order_divergence of the current velocities (interior, field precision). */
+int pycles_divergence(const pycles_dims *dims, const pycles_ref *ref, const void *u,
+                      const void *v, const void *w, void *div, int precision, void *stream);
+
+/* ---- layout conversion ------------------------------------------------- */
+/* Reference ghosted [i][j][k] (nlg = n + 2gw in every direction, FP64) to
+ * device layout in the given precision, minus base (may be NULL). */
+int pycles_layout_ingress(const pycles_dims *dims, const double *ref_layout, void *dev,
+                          const double *base, int precision, void *stream);
+/* Device layout to the reference layout (x/y ghosts = periodic images),
+ * plus base (may be NULL). */
+int pycles_layout_egress(const pycles_dims *dims, const void *dev, double *ref_layout,
+                         const double *base, int precision, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

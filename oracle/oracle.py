@@ -1,0 +1,295 @@
+"""CPU oracle for the PyCLES dynamical core (ctypes face of liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module.  Product code under ``pycles_amd/`` never does.
+
+Arrays are numpy float64 in the reference layout: one variable is
+``[nlg0, nlg1, nlg2]`` C-order, z fastest (reference
+``PrognosticVariables.pxd:45-46``).  ``OracleSimulation.stage`` follows the
+per-stage call order of ``Simulation3d.run`` (``Simulation3d.pyx:158-180``)
+restricted to the hot path: Th.update, SA.update, MA.update, TS.update,
+PV.Update_all_bcs, Pr.update, TS.adjust_timestep.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class OrcDims(C.Structure):
+    _fields_ = [("n", C.c_long * 3), ("nlg", C.c_long * 3), ("gw", C.c_long),
+                ("npg", C.c_long), ("dx", C.c_double * 3), ("dxi", C.c_double * 3)]
+
+
+class OrcLookup(C.Structure):
+    _fields_ = [("n", C.c_long), ("x_min", C.c_double), ("dx", C.c_double),
+                ("dxi", C.c_double), ("y", _dp)]
+
+
+class OrcRef(C.Structure):
+    _fields_ = [(name, _dp) for name in ("p0", "p0_half", "alpha0", "alpha0_half", "rho0", "rho0_half")]
+
+
+class OrcSAStats(C.Structure):
+    _fields_ = [("saturated", C.c_long), ("iterations", C.c_long),
+                ("max_iterations", C.c_long), ("capped", C.c_long)]
+
+
+def build():
+    """Compile liboracle.so from oracle.c with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        d = C.c_double
+        L.orc_interp_weno5.restype = d
+        L.orc_interp_weno7.restype = d
+        L.orc_interp_weno9.restype = d
+        L.orc_interp_4.restype = d
+        L.orc_interp_6.restype = d
+        L.orc_interp_8.restype = d
+        for f in ("orc_interp_weno5", "orc_interp_weno7", "orc_interp_weno9",
+                  "orc_interp_4", "orc_interp_6", "orc_interp_8"):
+            getattr(L, f).argtypes = [_dp]
+        L.orc_latent_heat_variable.restype = d
+        L.orc_latent_heat_variable.argtypes = [d]
+        L.orc_lookup_eval.restype = d
+        L.orc_lookup_eval.argtypes = [C.POINTER(OrcLookup), d]
+        L.orc_eos_sa.restype = C.c_int
+        L.orc_eos_sa.argtypes = [C.POINTER(OrcLookup), d, d, d, _dp, _dp, _dp, _dp, C.c_int]
+        L.orc_entropy_sa.restype = d
+        L.orc_entropy_sa.argtypes = [d, d, d, d, d]
+        L.orc_alpha.restype = d
+        L.orc_alpha.argtypes = [d, d, d, d]
+        L.orc_exner.restype = d
+        L.orc_exner.argtypes = [d]
+        L.orc_eos_dry.restype = d
+        L.orc_eos_dry.argtypes = [d, d]
+        L.orc_entropy_dry.restype = d
+        L.orc_entropy_dry.argtypes = [d, d]
+        L.orc_cfl_max.restype = d
+        L.orc_rk3_update.argtypes = [C.c_long, C.c_int, d, _dp, _dp, _dp]
+        L.orc_tdma_solve.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp]
+        L.orc_fft.argtypes = [C.c_long, _dp, _dp, C.c_int]
+        L.orc_update_bcs.argtypes = [C.POINTER(OrcDims), _dp, d]
+        _lib = L
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def make_dims(nx, ny, nz, gw, dx, dy, dz) -> OrcDims:
+    d = OrcDims()
+    lib().orc_dims_init(C.byref(d), C.c_long(nx), C.c_long(ny), C.c_long(nz), C.c_long(gw),
+                        C.c_double(dx), C.c_double(dy), C.c_double(dz))
+    return d
+
+
+class Lookup:
+    """Holds an ``OrcLookup`` over a pv*(T) table (reference lookup.h:16-41)."""
+
+    def __init__(self, T: np.ndarray, pv: np.ndarray):
+        self.y = np.ascontiguousarray(pv, dtype=np.float64)
+        self.c = OrcLookup()
+        self.c.n = len(self.y)
+        self.c.x_min = float(np.min(T))
+        self.c.dx = float(T[1] - T[0])
+        self.c.dxi = 1.0 / self.c.dx
+        self.c.y = ptr(self.y)
+
+    def __call__(self, x: float) -> float:
+        return lib().orc_lookup_eval(C.byref(self.c), C.c_double(x))
+
+
+class Ref:
+    """Reference-state profiles (length nlg[2]) bundled as ``OrcRef``."""
+
+    def __init__(self, p0, p0_half, alpha0, alpha0_half, rho0, rho0_half):
+        self.arrays = [np.ascontiguousarray(a, dtype=np.float64)
+                       for a in (p0, p0_half, alpha0, alpha0_half, rho0, rho0_half)]
+        self.c = OrcRef(*[ptr(a) for a in self.arrays])
+
+
+def weno(order: int, q) -> float:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    f = {5: lib().orc_interp_weno5, 7: lib().orc_interp_weno7, 9: lib().orc_interp_weno9}[order]
+    return f(ptr(q))
+
+
+def centred(order: int, q) -> float:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    f = {5: lib().orc_interp_4, 7: lib().orc_interp_6, 9: lib().orc_interp_8}[order]
+    return f(ptr(q))
+
+
+def eos_sa(lt: Lookup, p0, s, qt, max_iter=0):
+    T, qv, ql, qi = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+    it = lib().orc_eos_sa(C.byref(lt.c), p0, s, qt, C.byref(T), C.byref(qv), C.byref(ql),
+                          C.byref(qi), max_iter)
+    return T.value, qv.value, ql.value, qi.value, it
+
+
+def tdma(x, a, b, c):
+    x = np.array(x, dtype=np.float64)
+    scr = np.zeros_like(x)
+    lib().orc_tdma_solve(len(x), ptr(x), ptr(np.asarray(a, np.float64)), ptr(np.asarray(b, np.float64)),
+                         ptr(np.asarray(c, np.float64)), ptr(scr))
+    return x
+
+
+def fft(x, inverse=False):
+    x = np.asarray(x, dtype=np.complex128)
+    re = np.ascontiguousarray(x.real.copy())
+    im = np.ascontiguousarray(x.imag.copy())
+    lib().orc_fft(len(re), ptr(re), ptr(im), 1 if inverse else 0)
+    return re + 1j * im
+
+
+MOIST_PV = ("u", "v", "w", "s", "qt")
+DRY_PV = ("u", "v", "w", "s")
+MOIST_DV = ("temperature", "qv", "ql", "qi", "alpha", "buoyancy", "theta_rho",
+            "buoyancy_frequency", "thetali", "dynamic_pressure", "divergence")
+DRY_DV = ("temperature", "alpha", "buoyancy", "theta", "buoyancy_frequency",
+          "dynamic_pressure", "divergence")
+BC = {"u": 1.0, "v": 1.0, "w": -1.0, "s": 1.0, "qt": 1.0}
+
+
+class OracleSimulation:
+    """Reference-layout FP64 state plus the hot-path stage sequence."""
+
+    def __init__(self, dims: OrcDims, ref: Ref, moist: bool, lookup: Lookup | None,
+                 scalar_order=5, momentum_order=5, dt=1.0, dt_max=None, cfl_limit=1e6,
+                 t_max=1e30, max_iter=0):
+        self.d = dims
+        self.ref = ref
+        self.moist = moist
+        self.lt = lookup
+        self.so = scalar_order
+        self.mo = momentum_order
+        self.shape = tuple(dims.nlg)
+        self.npg = dims.npg
+        self.pv_names = MOIST_PV if moist else DRY_PV
+        self.dv_names = MOIST_DV if moist else DRY_DV
+        nv = len(self.pv_names)
+        self.values = np.zeros((nv,) + self.shape)
+        self.tend = np.zeros_like(self.values)
+        self.copy = np.zeros_like(self.values)
+        self.dv = {n: np.zeros(self.shape) for n in self.dv_names}
+        self.flux = np.zeros((3,) + self.shape)
+        self.dt = dt
+        self.dt_max = dt if dt_max is None else dt_max
+        self.cfl_limit = cfl_limit
+        self.t = 0.0
+        self.t_max = t_max
+        self.max_iter = max_iter
+        self.cfl_max = 0.0
+        self.sa_stats = OrcSAStats()
+
+    # -- accessors ---------------------------------------------------------
+    def pv(self, name):
+        return self.values[self.pv_names.index(name)]
+
+    def pt(self, name):
+        return self.tend[self.pv_names.index(name)]
+
+    # -- operators (each mirrors one reference update) ---------------------
+    def update_all_bcs(self):
+        for n in self.pv_names:
+            lib().orc_update_bcs(C.byref(self.d), ptr(self.pv(n)), BC[n])
+
+    def thermo_update(self):
+        L = lib()
+        r = C.byref(self.ref.c)
+        if self.moist:
+            dv = self.dv
+            L.orc_thermo_sa_update(C.byref(self.d), r, C.byref(self.lt.c), ptr(self.pv("s")),
+                                   ptr(self.pv("qt")), ptr(self.pt("w")), ptr(dv["temperature"]),
+                                   ptr(dv["qv"]), ptr(dv["ql"]), ptr(dv["qi"]), ptr(dv["alpha"]),
+                                   ptr(dv["buoyancy"]), ptr(dv["theta_rho"]),
+                                   ptr(dv["buoyancy_frequency"]), ptr(dv["thetali"]), 1,
+                                   self.max_iter, C.byref(self.sa_stats))
+        else:
+            dv = self.dv
+            L.orc_thermo_dry_update(C.byref(self.d), r, ptr(self.pv("s")), ptr(self.pt("w")),
+                                    ptr(dv["temperature"]), ptr(dv["alpha"]), ptr(dv["buoyancy"]),
+                                    ptr(dv["theta"]), ptr(dv["buoyancy_frequency"]))
+
+    def scalar_advection_update(self):
+        names = [n for n in self.pv_names if n in ("s", "qt")]
+        vel = (_dp * 3)(*[ptr(self.pv(n)) for n in ("u", "v", "w")])
+        phi = (_dp * len(names))(*[ptr(self.pv(n)) for n in names])
+        tend = (_dp * len(names))(*[ptr(self.pt(n)) for n in names])
+        lib().orc_scalar_advection_update(C.byref(self.d), C.byref(self.ref.c), vel, len(names), phi,
+                                          tend, ptr(self.flux), self.so)
+
+    def momentum_advection_update(self):
+        vel = (_dp * 3)(*[ptr(self.pv(n)) for n in ("u", "v", "w")])
+        tend = (_dp * 3)(*[ptr(self.pt(n)) for n in ("u", "v", "w")])
+        lib().orc_momentum_advection_update(C.byref(self.d), C.byref(self.ref.c), vel, tend, self.mo,
+                                            ptr(self.flux))
+
+    def rk_update(self, rk_step):
+        lib().orc_rk3_update(self.values.size, rk_step, self.dt, ptr(self.values), ptr(self.tend),
+                             ptr(self.copy))
+        if rk_step == 2:
+            self.t += self.dt
+
+    def pressure_update(self):
+        lib().orc_pressure_update(C.byref(self.d), C.byref(self.ref.c), ptr(self.pv("u")),
+                                  ptr(self.pv("v")), ptr(self.pv("w")),
+                                  ptr(self.dv["dynamic_pressure"]), ptr(self.dv["divergence"]))
+
+    def adjust_timestep(self):
+        """TimeStepping.adjust_timestep on the last stage (TimeStepping.pyx:94-108)."""
+        L = lib()
+        L.orc_cfl_max.argtypes = [C.POINTER(OrcDims), _dp, _dp, _dp, C.c_double]
+        self.cfl_max = L.orc_cfl_max(C.byref(self.d), ptr(self.pv("u")), ptr(self.pv("v")),
+                                     ptr(self.pv("w")), self.dt)
+        self.dt = min(self.dt_max, self.cfl_limit / (self.cfl_max / self.dt))
+        if self.t + self.dt > self.t_max:
+            self.dt = self.t_max - self.t
+
+    def stage(self, rk_step):
+        self.thermo_update()
+        self.scalar_advection_update()
+        self.momentum_advection_update()
+        self.rk_update(rk_step)
+        self.update_all_bcs()
+        self.pressure_update()
+        if rk_step == 2:
+            self.adjust_timestep()
+
+    def begin_run(self):
+        """The two calls ahead of the loop in Simulation3d.run (:141-148): the
+        boundary fill and a thermodynamics update.  The latter adds buoyancy
+        to the (not yet zeroed) w tendency, which the first stage then keeps."""
+        self.update_all_bcs()
+        self.thermo_update()
+
+    def step(self):
+        for rk in range(3):
+            self.stage(rk)
+
+    def divergence(self):
+        div = np.zeros(self.shape)
+        lib().orc_divergence(C.byref(self.d), C.byref(self.ref.c), ptr(self.pv("u")), ptr(self.pv("v")),
+                             ptr(self.pv("w")), ptr(div))
+        return div

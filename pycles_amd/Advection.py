@@ -1,0 +1,85 @@
+"""Scalar and momentum advection operators (device).
+
+``ScalarAdvection`` mirrors reference ``ScalarAdvection.pyx`` (constructor
+reads ``scalar_transport.order``; ``update(Gr, RS, PV, DV, Pa)`` adds
+-alpha0_half * d(flux)/dx for every scalar in x, y, z order).
+``MomentumAdvection`` mirrors ``MomentumAdvection.pyx`` (``momentum_transport.order``;
+``update(Gr, RS, PV, Pa)`` adds the nine (advected, advecting) pairs).
+Both call ``pycles_advection``; ``advect_fused`` issues one call for both,
+which gives bit-identical tendencies with one pass over the velocities.
+Orders 5, 7, 9 are supported; other orders raise (the reference silently
+falls back to WENO5 for scalars, scalar_advection.h:722-724, and to second
+order for momentum, momentum_advection.h:87-91).
+"""
+from __future__ import annotations
+
+from . import _capi
+from . import _device as D
+
+SUPPORTED = (5, 7, 9)
+
+
+def _scalar_names(PV):
+    return [n for n, t in zip(PV.index_name, PV.var_type) if t == 1]
+
+
+def _vel(PV):
+    return [PV.field(PV.velocity_names_directional[d]).data_ptr() for d in range(3)]
+
+
+def _base(PV, names):
+    if PV.precision != D.F32 or not names:
+        return None
+    b = [PV.base_ptr(n) for n in names] + [None, None]
+    return _capi.ScalarBase(b[0], b[1])
+
+
+class ScalarAdvection:
+    def __init__(self, namelist, LH=None, Pa=None):
+        try:
+            self.order = int(namelist["scalar_transport"]["order"])
+        except KeyError:
+            raise ValueError("scalar_transport order not given in namelist")
+        if self.order not in SUPPORTED:
+            raise ValueError(f"scalar_transport order {self.order} not supported (5, 7, 9)")
+
+    def initialize(self, Gr, PV, NS=None, Pa=None):
+        self.dims = D.dims_of(Gr)
+
+    def update(self, Gr, RS, PV, DV=None, Pa=None):
+        names = _scalar_names(PV)
+        if not names:
+            return
+        ref = D.device_ref(RS)
+        _capi.advection(self.dims, ref.c, _vel(PV), None, [PV.ptr(n) for n in names],
+                        [PV.tptr(n) for n in names], _base(PV, names), self.order, self.order,
+                        PV.precision, D.stream())
+
+
+class MomentumAdvection:
+    def __init__(self, namelist, Pa=None):
+        try:
+            self.order = int(namelist["momentum_transport"]["order"])
+        except KeyError:
+            raise ValueError("momentum_transport order not given in namelist")
+        if self.order not in SUPPORTED:
+            raise ValueError(f"momentum_transport order {self.order} not supported (5, 7, 9)")
+
+    def initialize(self, Gr, PV, NS=None, Pa=None):
+        self.dims = D.dims_of(Gr)
+
+    def update(self, Gr, RS, PV, Pa=None):
+        ref = D.device_ref(RS)
+        tv = [PV.tptr(PV.velocity_names_directional[d]) for d in range(3)]
+        _capi.advection(self.dims, ref.c, _vel(PV), tv, [], [], None, self.order, self.order,
+                        PV.precision, D.stream())
+
+
+def advect_fused(SA: ScalarAdvection, MA: MomentumAdvection, Gr, RS, PV):
+    """SA.update followed by MA.update in one launch sequence."""
+    names = _scalar_names(PV)
+    ref = D.device_ref(RS)
+    tv = [PV.tptr(PV.velocity_names_directional[d]) for d in range(3)]
+    _capi.advection(SA.dims, ref.c, _vel(PV), tv, [PV.ptr(n) for n in names],
+                    [PV.tptr(n) for n in names], _base(PV, names), SA.order, MA.order,
+                    PV.precision, D.stream())

@@ -1,0 +1,164 @@
+"""Prognostic and diagnostic state in HBM.
+
+Mirrors reference ``PrognosticVariables.pyx`` / ``DiagnosticVariables.pyx``:
+variables are registered by name with a boundary type ("sym" -> 1.0,
+"asym" -> -1.0) and, for PV, a variable type (velocity / scalar); the values
+and tendencies live in one flat device buffer with variable v at offset
+``v * npg`` (``get_varshift``).  The difference is the per-variable layout:
+``[k][j][i]`` over ``nzg`` levels and the interior columns (see ``Grid``),
+in the run's precision.
+
+In FP32 runs the scalars s and qt are stored as deviations from an FP64
+level profile (``base``), taken as the horizontal mean of the initial field;
+``get_variable_array`` and ``set_variable_array`` add and remove it.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _device as D
+
+
+class _State:
+    def _register(self, name, bc_type, Pa=None):
+        if bc_type == "sym":
+            bc = 1.0
+        elif bc_type == "asym":
+            bc = -1.0
+        else:
+            raise ValueError(f"Not a valid bc_type {bc_type!r}")
+        self.name_index[name] = self.nv
+        self.index_name.append(name)
+        self.bc_type.append(bc)
+        self.nv = len(self.name_index)
+
+    def get_nv(self, name):
+        return self.name_index[name]
+
+    def get_varshift(self, Gr, name):
+        return self.name_index[name] * Gr.dims.npdev
+
+    def field(self, name) -> torch.Tensor:
+        return self.values[self.name_index[name]]
+
+    def ptr(self, name) -> int:
+        return self.field(name).data_ptr()
+
+
+class PrognosticVariables(_State):
+    def __init__(self, Gr):
+        self.name_index = {}
+        self.index_name = []
+        self.units = {}
+        self.bc_type = []
+        self.var_type = []
+        self.nv = 0
+        self.nv_scalars = 0
+        self.nv_velocities = 0
+        self.velocity_directions = [0, 0, 0]
+        self.velocity_names_directional = ["", "", ""]
+        self.base = {}
+        self.precision = D.F64
+
+    def add_variable(self, name, units, nice_name, desc, bc_type, var_type, Pa=None):
+        self._register(name, bc_type, Pa)
+        self.units[name] = units
+        if var_type == "velocity":
+            self.var_type.append(0)
+            self.nv_velocities += 1
+        elif var_type == "scalar":
+            self.var_type.append(1)
+            self.nv_scalars += 1
+        else:
+            raise ValueError(f"Not a valid var_type {var_type!r}")
+
+    def set_velocity_direction(self, name, direction, Pa=None):
+        self.velocity_directions[direction] = self.get_nv(name)
+        self.velocity_names_directional[direction] = name
+
+    def initialize(self, Gr, NS=None, Pa=None, precision=D.F64):
+        self.precision = D.precision_code(precision)
+        dev = D.require_gpu()
+        d = Gr.dims
+        shape = (self.nv, d.nzg, d.n[1], d.n[0])
+        dt = D.torch_dtype(self.precision)
+        self.values = torch.zeros(shape, dtype=dt, device=dev)
+        self.tendencies = torch.zeros(shape, dtype=dt, device=dev)
+        self.Gr = Gr
+
+    def tend(self, name) -> torch.Tensor:
+        return self.tendencies[self.name_index[name]]
+
+    def tptr(self, name) -> int:
+        return self.tend(name).data_ptr()
+
+    def base_ptr(self, name):
+        b = self.base.get(name)
+        return None if b is None else b.data_ptr()
+
+    # ---- host <-> device in the reference layout -------------------------
+    def set_base_from_device_layout(self, name, dev_layout: np.ndarray):
+        """FP32 runs: base(k) = horizontal mean of the given field."""
+        if self.precision == D.F32:
+            prof = np.asarray(dev_layout, dtype=np.float64).mean(axis=(1, 2))
+            self.base[name] = D.to_device(prof)
+
+    def set_device_layout(self, name, arr: np.ndarray):
+        """Upload a field given in device layout [k][j][i] (FP64 host array)."""
+        t = torch.from_numpy(np.ascontiguousarray(arr, dtype=np.float64)).to(self.values.device)
+        if name in self.base:
+            t = t - self.base[name][:, None, None]
+        self.field(name).copy_(t.to(self.values.dtype))
+
+    def get_device_layout(self, name) -> np.ndarray:
+        t = self.field(name).to(torch.float64)
+        if name in self.base:
+            t = t + self.base[name][:, None, None]
+        return t.cpu().numpy()
+
+    def set_variable_array(self, name, Gr, ref_layout: np.ndarray):
+        self.set_device_layout(name, Gr.to_device_layout(ref_layout))
+
+    def get_variable_array(self, name, Gr) -> np.ndarray:
+        """Reference ghosted [i][j][k] layout, FP64 (reference ``get_variable_array``)."""
+        return Gr.from_device_layout(self.get_device_layout(name))
+
+    def get_tendency_array(self, name, Gr) -> np.ndarray:
+        return Gr.from_device_layout(self.tend(name).to(torch.float64).cpu().numpy())
+
+    def zero_tendencies(self):
+        self.tendencies.zero_()
+
+    def val_nan(self, PA=None, message=""):
+        if not torch.isfinite(self.values).all():
+            raise FloatingPointError("Nans found in Prognostic Variables values " + message)
+
+    def tend_nan(self, PA=None, message=""):
+        if not torch.isfinite(self.tendencies).all():
+            raise FloatingPointError("Nans found in tendencies " + message)
+
+
+class DiagnosticVariables(_State):
+    def __init__(self):
+        self.name_index = {}
+        self.index_name = []
+        self.units = {}
+        self.bc_type = []
+        self.nv = 0
+
+    def add_variables(self, name, units, nice_name, desc, bc_type, Pa=None):
+        self._register(name, bc_type, Pa)
+        self.units[name] = units
+
+    def initialize(self, Gr, NS=None, Pa=None, precision=D.F64):
+        self.precision = D.precision_code(precision)
+        d = Gr.dims
+        self.values = torch.zeros((self.nv, d.nzg, d.n[1], d.n[0]),
+                                  dtype=D.torch_dtype(self.precision), device=D.require_gpu())
+
+    def get_device_layout(self, name) -> np.ndarray:
+        return self.field(name).to(torch.float64).cpu().numpy()
+
+    def get_variable_array(self, name, Gr) -> np.ndarray:
+        return Gr.from_device_layout(self.get_device_layout(name))

@@ -1,0 +1,115 @@
+"""Driver of the device dynamical core (reference ``Simulation3d.pyx``).
+
+``initialize(namelist)`` builds the operators in the reference's order
+(``Simulation3d.pyx:40-138``) restricted to the hot path; ``run()`` follows
+``:140-189``: one boundary fill and one thermodynamics update before the loop
+(the latter adds buoyancy to the w tendency, which the first stage keeps), then
+for each RK stage Th.update, SA.update, MA.update, TS.update (with the z
+boundary fill), Pr.update and, on the last stage, TS.adjust_timestep.  The
+operators outside the hot path (kinematics, SGS, diffusion, surface, forcing,
+radiation, damping, tracers, microphysics processes, IO) are not part of
+this build and contribute nothing, as with a namelist that disables them.
+"""
+from __future__ import annotations
+
+import time
+
+from . import _device as D
+from .Advection import MomentumAdvection, ScalarAdvection, advect_fused
+from .Grid import Grid
+from .ParallelMPI import NetCDFIO_Stats, ParallelMPI
+from .PressureSolver import PressureSolver
+from .PrognosticVariables import DiagnosticVariables, PrognosticVariables
+from .ReferenceState import ReferenceState
+from .Thermodynamics import LatentHeat
+from .ThermodynamicsSA import ThermodynamicsFactory
+from .TimeStepping import TimeStepping
+from . import cases
+
+
+class Simulation3d:
+    def __init__(self, namelist=None):
+        self.fuse_advection = True
+
+    def initialize(self, namelist, Gr=None, Ref=None, Th=None, fields=None):
+        """Build the run.  ``Gr``, ``Ref``, ``Th`` and ``fields`` (device-layout
+        FP64 host arrays per prognostic name) may be passed in to start from an
+        already built state instead of running the case initializer."""
+        self.namelist = namelist
+        precision = namelist.get("mi355x", {}).get("precision", "f64")
+        self.Pa = ParallelMPI(namelist)
+        self.Gr = Gr if Gr is not None else Grid(namelist, self.Pa)
+        self.PV = PrognosticVariables(self.Gr)
+        self.DV = DiagnosticVariables()
+        self.Pr = PressureSolver()
+        self.LH = LatentHeat(namelist, self.Pa)
+        self.SA = ScalarAdvection(namelist, self.LH, self.Pa)
+        self.MA = MomentumAdvection(namelist, self.Pa)
+        self.Th = Th if Th is not None else ThermodynamicsFactory(namelist, None, self.LH, self.Pa)
+        self.Ref = Ref if Ref is not None else ReferenceState(self.Gr)
+        self.TS = TimeStepping()
+        self.StatsIO = NetCDFIO_Stats()
+        self.PV.add_variable("u", "m/s", "u", "u velocity component", "sym", "velocity", self.Pa)
+        self.PV.set_velocity_direction("u", 0, self.Pa)
+        self.PV.add_variable("v", "m/s", "v", "v velocity component", "sym", "velocity", self.Pa)
+        self.PV.set_velocity_direction("v", 1, self.Pa)
+        self.PV.add_variable("w", "m/s", "w", "w velocity component", "asym", "velocity", self.Pa)
+        self.PV.set_velocity_direction("w", 2, self.Pa)
+        self.Th.initialize(self.Gr, self.PV, self.DV, self.StatsIO, self.Pa)
+        self.PV.initialize(self.Gr, self.StatsIO, self.Pa, precision=precision)
+        self.SA.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
+        self.MA.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
+        self.TS.initialize(namelist, self.PV, self.Pa)
+        if fields is None:
+            fields = cases.initialize(namelist, self.Gr, self.PV, self.Ref, self.Th)
+        else:
+            for name, arr in fields.items():
+                if name in ("s", "qt"):
+                    self.PV.set_base_from_device_layout(name, arr)
+                self.PV.set_device_layout(name, arr)
+        self.initial_fields = fields
+        self.Pr.initialize(namelist, self.Gr, self.Ref, self.DV, self.Pa)
+        self.DV.initialize(self.Gr, self.StatsIO, self.Pa, precision=precision)
+        self.walltimes = []
+        return self
+
+    # ---------------------------------------------------------------------
+    def begin_run(self):
+        """Simulation3d.run before its loop: boundary fill, thermodynamics."""
+        self.TS._zbcs = D.dims_of(self.Gr)
+        from . import _capi
+        names = self.PV.index_name
+        _capi.apply_z_bcs(self.TS._zbcs, [self.PV.ptr(n) for n in names],
+                          [self.PV.bc_type[self.PV.get_nv(n)] for n in names], self.PV.precision,
+                          D.stream())
+        self.Th.update(self.Gr, self.Ref, self.PV, self.DV)
+
+    def stage(self, rk_step):
+        self.TS.rk_step = rk_step
+        self.Th.update(self.Gr, self.Ref, self.PV, self.DV)
+        if self.fuse_advection:
+            advect_fused(self.SA, self.MA, self.Gr, self.Ref, self.PV)
+        else:
+            self.SA.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa)
+            self.MA.update(self.Gr, self.Ref, self.PV, self.Pa)
+        self.TS.update(self.Gr, self.PV, self.Pa)
+        last = rk_step == self.TS.n_rk_steps - 1
+        self.Pr.update(self.Gr, self.Ref, self.DV, self.PV, self.Pa,
+                       cfl_out=self.TS.cfl_buffer() if last else None, dt=self.TS.dt)
+        self.TS.adjust_timestep(self.Gr, self.PV, self.DV, self.Pa)
+
+    def step(self):
+        for rk in range(self.TS.n_rk_steps):
+            self.stage(rk)
+
+    def run(self, max_steps=None):
+        self.begin_run()
+        n = 0
+        while self.TS.t < self.TS.t_max and (max_steps is None or n < max_steps):
+            t0 = time.time()
+            self.step()
+            self.walltimes.append(time.time() - t0)
+            self.Pa.root_print(f"T = {self.TS.t} dt = {self.TS.dt} cfl_max = {self.TS.cfl_max} "
+                               f"walltime = {self.walltimes[-1]}")
+            n += 1
+        return self

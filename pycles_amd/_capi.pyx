@@ -1,0 +1,271 @@
+# cython: language_level=3, boundscheck=False, wraparound=False
+"""Thin Cython binding of the C ABI in include/pycles_amd.h.
+
+Device pointers cross this layer as Python ints (``tensor.data_ptr()``) and
+the stream as the integer handle of a HIP stream.  Every call checks the
+returned status and raises ``PyclesError`` with the library's message.  This
+is the counterpart of the reference's ``cdef extern from "<header>.h"``
+declarations (e.g. ScalarAdvection.pyx:23-29, MomentumAdvection.pyx:17-21,
+ThermodynamicsSA.pyx:21-29).
+"""
+from libc.stdint cimport int64_t, uintptr_t
+from libc.stdlib cimport malloc, free
+
+cdef extern from "pycles_amd.h":
+    ctypedef struct pycles_dims:
+        int64_t n[3]
+        int64_t gw
+        double dx[3]
+        double dxi[3]
+    ctypedef struct pycles_ref:
+        const double *rho0
+        const double *rho0_half
+        const double *alpha0
+        const double *alpha0_half
+        const double *p0
+        const double *p0_half
+    ctypedef struct pycles_scalar_base:
+        const double *base[2]
+    ctypedef struct pycles_lookup:
+        const double *y
+        int64_t n
+        double x_min
+        double dx
+        double dxi
+    ctypedef struct pycles_sa_options:
+        int do_qt_clipping
+        int max_iter
+        int latent_heat
+        const void *mlp
+        int newton_steps
+        long long *counters
+    ctypedef struct pycles_pressure_plan:
+        pass
+
+    int PYCLES_F64
+    int PYCLES_F32
+    int PYCLES_LH_VARIABLE
+    int PYCLES_LH_CONSTANT
+
+    const char *pycles_last_error()
+    const char *pycles_version()
+    int pycles_device_count()
+    int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const *vel,
+                         void *const *tend_vel, int nscal, const void *const *scal,
+                         void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
+                         int momentum_order, int precision, void *stream)
+    int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt,
+                         void *s, void *qt, void *wt, void *T, void *qv, void *ql, void *alpha,
+                         void *buoyancy, void *theta_rho, void *bvf, void *thetali,
+                         const pycles_scalar_base *base, const pycles_sa_options *opt, int precision,
+                         void *stream)
+    int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref, void *s, void *wt, void *T,
+                          void *alpha, void *buoyancy, void *theta, void *bvf,
+                          const pycles_scalar_base *base, int precision, void *stream)
+    int pycles_rk3_stage(const pycles_dims *dims, int nv, void *const *values, void *const *tend,
+                         void *const *copies, const double *bc_type, int rk_step, double dt,
+                         int precision, void *stream)
+    int pycles_apply_z_bcs(const pycles_dims *dims, int nv, void *const *values,
+                           const double *bc_type, int precision, void *stream)
+    int pycles_level_mean(const pycles_dims *dims, const void *field, double *mean, int precision,
+                          void *stream)
+    int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
+                       double dt, double *out, int precision, void *stream)
+    int pycles_pressure_plan_create(const pycles_dims *dims, const double *rho0_host,
+                                    const double *rho0_half_host, int precision,
+                                    pycles_pressure_plan **plan)
+    void pycles_pressure_plan_destroy(pycles_pressure_plan *plan)
+    int pycles_pressure_update(pycles_pressure_plan *plan, const pycles_ref *ref, void *u, void *v,
+                               void *w, void *p, void *div, double dt, double *cfl_out, void *stream)
+    int pycles_divergence(const pycles_dims *dims, const pycles_ref *ref, const void *u,
+                          const void *v, const void *w, void *div, int precision, void *stream)
+    int pycles_layout_ingress(const pycles_dims *dims, const double *ref_layout, void *dev,
+                              const double *base, int precision, void *stream)
+    int pycles_layout_egress(const pycles_dims *dims, const void *dev, double *ref_layout,
+                             const double *base, int precision, void *stream)
+
+F64 = PYCLES_F64
+F32 = PYCLES_F32
+LH_VARIABLE = PYCLES_LH_VARIABLE
+LH_CONSTANT = PYCLES_LH_CONSTANT
+
+
+class PyclesError(RuntimeError):
+    pass
+
+
+cdef inline void _check(int rc) except *:
+    if rc != 0:
+        raise PyclesError("pycles_amd error %d: %s" % (rc, pycles_last_error().decode()))
+
+
+cdef inline void *_p(object x):
+    if x is None:
+        return NULL
+    return <void *> <uintptr_t> x
+
+
+def version():
+    return pycles_version().decode()
+
+
+def device_count():
+    return pycles_device_count()
+
+
+cdef class Dims:
+    cdef pycles_dims d
+
+    def __init__(self, n, gw, dx):
+        for i in range(3):
+            self.d.n[i] = n[i]
+            self.d.dx[i] = dx[i]
+            self.d.dxi[i] = 1.0 / dx[i]
+        self.d.gw = gw
+
+
+cdef class Ref:
+    cdef pycles_ref r
+
+    def __init__(self, rho0, rho0_half, alpha0, alpha0_half, p0, p0_half):
+        self.r.rho0 = <const double *> _p(rho0)
+        self.r.rho0_half = <const double *> _p(rho0_half)
+        self.r.alpha0 = <const double *> _p(alpha0)
+        self.r.alpha0_half = <const double *> _p(alpha0_half)
+        self.r.p0 = <const double *> _p(p0)
+        self.r.p0_half = <const double *> _p(p0_half)
+
+
+cdef class ScalarBase:
+    cdef pycles_scalar_base b
+
+    def __init__(self, b0=None, b1=None):
+        self.b.base[0] = <const double *> _p(b0)
+        self.b.base[1] = <const double *> _p(b1)
+
+
+cdef class Lookup:
+    cdef pycles_lookup t
+
+    def __init__(self, y_ptr, n, x_min, dx, dxi):
+        self.t.y = <const double *> _p(y_ptr)
+        self.t.n = n
+        self.t.x_min = x_min
+        self.t.dx = dx
+        self.t.dxi = dxi
+
+
+cdef class SAOptions:
+    cdef pycles_sa_options o
+
+    def __init__(self, do_qt_clipping=True, max_iter=100, latent_heat=0, mlp=None,
+                 newton_steps=0, counters=None):
+        self.o.do_qt_clipping = 1 if do_qt_clipping else 0
+        self.o.max_iter = max_iter
+        self.o.latent_heat = latent_heat
+        self.o.mlp = _p(mlp)
+        self.o.newton_steps = newton_steps
+        self.o.counters = <long long *> _p(counters)
+
+
+cdef class PressurePlan:
+    cdef pycles_pressure_plan *plan
+
+    def __cinit__(self):
+        self.plan = NULL
+
+    def __init__(self, Dims dims, rho0_host_ptr, rho0_half_host_ptr, int precision):
+        _check(pycles_pressure_plan_create(&dims.d, <const double *> _p(rho0_host_ptr),
+                                           <const double *> _p(rho0_half_host_ptr), precision,
+                                           &self.plan))
+
+    def __dealloc__(self):
+        if self.plan != NULL:
+            pycles_pressure_plan_destroy(self.plan)
+            self.plan = NULL
+
+    def update(self, Ref ref, u, v, w, p, div, double dt, cfl_out, stream):
+        _check(pycles_pressure_update(self.plan, &ref.r, _p(u), _p(v), _p(w), _p(p), _p(div), dt,
+                                      <double *> _p(cfl_out), _p(stream)))
+
+
+def advection(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase base,
+              int scalar_order, int momentum_order, int precision, stream):
+    cdef const void *v3[3]
+    cdef void *t3[3]
+    cdef const void *sc[2]
+    cdef void *ts[2]
+    cdef int ns = len(scal)
+    for i in range(3):
+        v3[i] = _p(vel[i])
+        t3[i] = _p(tend_vel[i]) if tend_vel is not None else NULL
+    for i in range(ns):
+        sc[i] = _p(scal[i])
+        ts[i] = _p(tend_scal[i])
+    _check(pycles_advection(&dims.d, &ref.r, v3, t3 if tend_vel is not None else NULL, ns, sc, ts,
+                            &base.b if base is not None else NULL, scalar_order, momentum_order,
+                            precision, _p(stream)))
+
+
+def thermo_sa(Dims dims, Ref ref, Lookup lt, s, qt, wt, T, qv, ql, alpha, buoyancy, theta_rho,
+              bvf, thetali, ScalarBase base, SAOptions opt, int precision, stream):
+    _check(pycles_thermo_sa(&dims.d, &ref.r, &lt.t, _p(s), _p(qt), _p(wt), _p(T), _p(qv), _p(ql),
+                            _p(alpha), _p(buoyancy), _p(theta_rho), _p(bvf), _p(thetali),
+                            &base.b if base is not None else NULL, &opt.o, precision, _p(stream)))
+
+
+def thermo_dry(Dims dims, Ref ref, s, wt, T, alpha, buoyancy, theta, bvf, ScalarBase base,
+               int precision, stream):
+    _check(pycles_thermo_dry(&dims.d, &ref.r, _p(s), _p(wt), _p(T), _p(alpha), _p(buoyancy),
+                             _p(theta), _p(bvf), &base.b if base is not None else NULL, precision,
+                             _p(stream)))
+
+
+def rk3_stage(Dims dims, values, tend, copies, bc_type, int rk_step, double dt, int precision, stream):
+    cdef int nv = len(values)
+    cdef void *v[8]
+    cdef void *t[8]
+    cdef void *c[8]
+    cdef double bc[8]
+    if nv > 8:
+        raise ValueError("at most 8 fields")
+    for i in range(nv):
+        v[i] = _p(values[i])
+        t[i] = _p(tend[i])
+        c[i] = _p(copies[i])
+        bc[i] = bc_type[i]
+    _check(pycles_rk3_stage(&dims.d, nv, v, t, c, bc, rk_step, dt, precision, _p(stream)))
+
+
+def apply_z_bcs(Dims dims, values, bc_type, int precision, stream):
+    cdef int nv = len(values)
+    cdef void *v[8]
+    cdef double bc[8]
+    if nv > 8:
+        raise ValueError("at most 8 fields")
+    for i in range(nv):
+        v[i] = _p(values[i])
+        bc[i] = bc_type[i]
+    _check(pycles_apply_z_bcs(&dims.d, nv, v, bc, precision, _p(stream)))
+
+
+def level_mean(Dims dims, field, mean, int precision, stream):
+    _check(pycles_level_mean(&dims.d, _p(field), <double *> _p(mean), precision, _p(stream)))
+
+
+def cfl_max(Dims dims, u, v, w, double dt, out, int precision, stream):
+    _check(pycles_cfl_max(&dims.d, _p(u), _p(v), _p(w), dt, <double *> _p(out), precision, _p(stream)))
+
+
+def divergence(Dims dims, Ref ref, u, v, w, div, int precision, stream):
+    _check(pycles_divergence(&dims.d, &ref.r, _p(u), _p(v), _p(w), _p(div), precision, _p(stream)))
+
+
+def layout_ingress(Dims dims, ref_layout, dev, base, int precision, stream):
+    _check(pycles_layout_ingress(&dims.d, <const double *> _p(ref_layout), _p(dev),
+                                 <const double *> _p(base), precision, _p(stream)))
+
+
+def layout_egress(Dims dims, dev, ref_layout, base, int precision, stream):
+    _check(pycles_layout_egress(&dims.d, _p(dev), <double *> _p(ref_layout),
+                                <const double *> _p(base), precision, _p(stream)))

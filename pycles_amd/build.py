@@ -1,0 +1,88 @@
+"""Build the gfx950 library and its Cython binding in-tree.
+
+    python -m pycles_amd.build          (or __graft_entry__.build())
+
+1. ``hipcc`` compiles every ``csrc/*.hip`` for gfx950 into
+   ``pycles_amd/libpycles_amd.so`` (C ABI declared in ``include/pycles_amd.h``).
+2. Cython translates ``_capi.pyx`` and gcc builds ``pycles_amd/_capi*.so``,
+   linked against the library with an ``$ORIGIN`` rpath.
+No GPU is needed to build.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libpycles_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -ffp-contract=off: the FP64 path reproduces the reference's rounding, which
+# has no fused multiply-add.  Packed FP32 instructions are disabled.
+HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: " + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def _stale(target, deps):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_hip(force=True, jobs=8):
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    headers.append(os.path.join(ROOT, "include", "pycles_amd.h"))
+    objs = []
+    jobs_list = []
+    for s in srcs:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(BUILD, s[:-4] + ".o")
+        objs.append(obj)
+        if force or _stale(obj, [src] + headers):
+            jobs_list.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, jobs_list))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs)
+    return LIB
+
+
+def build_capi(force=True):
+    pyx = os.path.join(HERE, "_capi.pyx")
+    c_file = os.path.join(BUILD, "_capi.c")
+    ext = sysconfig.get_config_var("EXT_SUFFIX")
+    so = os.path.join(HERE, "_capi" + ext)
+    if force or _stale(so, [pyx, LIB, os.path.join(ROOT, "include", "pycles_amd.h")]):
+        os.makedirs(BUILD, exist_ok=True)
+        _run([sys.executable, "-m", "cython", "-3", "-o", c_file, pyx])
+        inc = sysconfig.get_paths()["include"]
+        _run(["gcc", "-O2", "-fPIC", "-shared", "-I", inc, "-I", os.path.join(ROOT, "include"),
+              "-D__HIP_PLATFORM_AMD__", c_file, "-o", so, "-L", HERE, "-lpycles_amd",
+              "-Wl,-rpath,$ORIGIN"])
+    return so
+
+
+def build(force=True):
+    build_hip(force=force)
+    build_capi(force=force)
+
+
+if __name__ == "__main__":
+    build(force="--incremental" not in sys.argv)
+    print("built", LIB)

@@ -1,0 +1,252 @@
+"""Seeded initial states for the benchmark configurations.
+
+The reference initializes each case on the host (e.g. ``InitStableBubble``,
+``Initialization.pyx:64-105``; ``InitBomex``, ``:260-368``) with numpy's
+global RNG.  Here the noise comes from a counter-based generator,
+SplitMix64(seed, field, cell index) -> U[0,1), with Box-Muller for normals, so
+a state is reproducible independent of layout and thread count.  Fields are
+built in the device layout ``[k][j][i]`` as FP64 numpy arrays, the z ghost
+levels are filled with the boundary rule of each variable, and the arrays are
+uploaded once.  White-noise velocities are strongly divergent, so the seeded
+velocities are projected onto the divergence-free space once before the run
+(``project_initial``); advecting entropy with the raw noise would change s by
+O(100) J/kg/K in the first stage.
+
+Configurations (sizes, spacings, profiles) follow the benchmark definitions:
+  dry64     64^3, dx=dy=100 m, dz=40 m, dry, theta=300 K + U(+-0.1 K) below
+            1 km, u=5+N(0,0.5), v,w=N(0,0.5)
+  moist     BOMEX-like moist profiles (qt 17 -> 16.3 g/kg at 520 m -> 4.3 g/kg
+            at 2 km, held above; theta_l 298.7 K to 520 m then +3.85 K/km;
+            qt +-0.025 g/kg, theta_l +-0.1 K uniform noise below 1.6 km;
+            u=-8.75+N(0,sigma), v,w=N(0,sigma)) on any grid.
+Profiles are evaluated at cell centres z_half (the reference's InitBomex
+evaluates them at ``Gr.zl_half``, which Grid.pyx:166-167 sets to full levels).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import parameters as P
+from . import thermo_host as H
+
+_G = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _splitmix64(x):
+    x = x + _G
+    z = (x ^ (x >> np.uint64(30))) * _M1
+    z = (z ^ (z >> np.uint64(27))) * _M2
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, field: int, shape):
+    """U[0,1) per cell of an interior block of the given (nz, ny, nx) shape."""
+    n = int(np.prod(shape))
+    with np.errstate(over="ignore"):
+        key = _splitmix64(np.uint64(seed) * _G ^ (np.uint64(field) << np.uint64(56)))
+        idx = np.arange(n, dtype=np.uint64)
+        z = _splitmix64(idx ^ key)
+    return ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)).reshape(shape)
+
+
+def normal(seed: int, field: int, shape):
+    u1 = 1.0 - uniform(seed, 2 * field, shape)
+    u2 = uniform(seed, 2 * field + 1, shape)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def fill_z_ghosts(f: np.ndarray, gw: int, bc: float):
+    """set_bcs in z on a device-layout array (prognostic_variables.h:205-270)."""
+    nzg = f.shape[0]
+    if bc == 1.0:
+        for m in range(gw):
+            f[gw - 1 - m] = f[gw + m]
+            f[nzg - gw + m] = f[nzg - gw - 1 - m]
+    else:
+        f[gw - 1] = 0.0
+        f[nzg - gw] = 0.0
+        for m in range(1, gw):
+            f[gw - 1 - m] = bc * f[gw - 1 + m]
+            f[nzg - gw + m] = bc * f[nzg - gw - m]
+    return f
+
+
+CONFIGS = {
+    "dry64": dict(nx=64, ny=64, nz=64, gw=3, dx=100.0, dy=100.0, dz=40.0, moist=False,
+                  order=(5, 5), dt=1.0, steps=1, seed=1, sigma=0.5),
+    "moist256": dict(nx=256, ny=256, nz=128, gw=3, dx=50.0, dy=50.0, dz=20.0, moist=True,
+                     order=(5, 5), dt=2.0, steps=10, seed=2, sigma=0.3),
+    "moist512": dict(nx=512, ny=512, nz=256, gw=3, dx=50.0, dy=50.0, dz=20.0, moist=True,
+                     order=(5, 5), dt=1.0, steps=100, seed=3, sigma=0.3),
+    "moist1024": dict(nx=1024, ny=1024, nz=192, gw=5, dx=25.0, dy=25.0, dz=10.0, moist=True,
+                      order=(9, 5), dt=0.5, steps=20, seed=5, sigma=1.0),
+}
+
+
+def namelist(nx, ny, nz, gw, dx, dy, dz, moist, order=(5, 5), dt=1.0, t_max=1e9, precision="f64",
+             cfl_limit=1e6, **extra):
+    nl = {
+        "grid": {"dims": 3, "nx": nx, "ny": ny, "nz": nz, "gw": gw, "dx": dx, "dy": dy, "dz": dz},
+        "scalar_transport": {"order": order[0]},
+        "momentum_transport": {"order": order[1]},
+        "time_stepping": {"ts_type": 3, "dt_initial": dt, "dt_max": dt, "t_max": t_max,
+                          "cfl_limit": cfl_limit},
+        "thermodynamics": {"do_qt_clipping": True},
+        "microphysics": {"scheme": "None_SA" if moist else "None_Dry"},
+        "mi355x": {"precision": precision},
+        "initialization": {"seed": extra.get("seed", 1), "sigma": extra.get("sigma", 0.5),
+                           "case": "moist" if moist else "dry"},
+    }
+    return nl
+
+
+def config_namelist(name, precision="f64", **over):
+    c = dict(CONFIGS[name])
+    c.update(over)
+    return namelist(precision=precision, **c)
+
+
+def _velocities(Gr, seed, sigma, u_mean):
+    d = Gr.dims
+    gw = d.gw
+    shape = (d.n[2], d.n[1], d.n[0])
+    out = {}
+    for f, (name, mean) in enumerate((("u", u_mean), ("v", 0.0), ("w", 0.0))):
+        a = np.zeros((d.nzg, d.n[1], d.n[0]))
+        a[gw:gw + d.n[2]] = mean + sigma * normal(seed, 10 + f, shape)
+        if name == "w":
+            a[gw + d.n[2] - 1] = 0.0  # top interior face w
+        out[name] = fill_z_ghosts(a, gw, -1.0 if name == "w" else 1.0)
+    return out
+
+
+def project_initial(Gr, RS, u, v, w):
+    """Make seeded velocity noise divergence-free before the run starts.
+
+    Uses the discrete operators of the pressure projection (second-order
+    divergence of rho0*u, PressureSolver.pyx:138-175; modified wavenumbers and
+    tridiagonal rows, PressureFFTParallel.pyx:64-148; (0,0) mode zero) with
+    numpy FFTs.  Initialization only: the per-stage projection runs on the GPU.
+    """
+    d = Gr.dims
+    gw, nz = d.gw, d.n[2]
+    dx, dy, dz = d.dx
+    sl = slice(gw, gw + nz)
+    ah = RS.alpha0_half[sl][:, None, None]
+    a0 = RS.alpha0[sl][:, None, None]
+    a0m = RS.alpha0[gw - 1:gw - 1 + nz][:, None, None]
+    w = w - w[:, :, :].mean(axis=(1, 2), keepdims=True)
+    ui, vi = u[sl], v[sl]
+    div = (ui / ah - np.roll(ui, 1, axis=2) / ah) / dx
+    div = div + (vi / ah - np.roll(vi, 1, axis=1) / ah) / dy
+    div = div + (w[sl] / a0 - w[gw - 1:gw - 1 + nz] / a0m) / dz
+    spec = np.fft.fft2(div, axes=(1, 2))
+    kx = np.arange(d.n[0]); kx = np.where(kx <= d.n[0] // 2, kx, kx - d.n[0])
+    ky = np.arange(d.n[1]); ky = np.where(ky <= d.n[1] // 2, ky, ky - d.n[1])
+    kx2 = (2.0 * np.cos(2.0 * np.pi / d.n[0] * kx) - 2.0) / dx / dx
+    ky2 = (2.0 * np.cos(2.0 * np.pi / d.n[1] * ky) - 2.0) / dy / dy
+    kk = ky2[:, None] + kx2[None, :]
+    rho0, rho0h = RS.rho0, RS.rho0_half
+    dzi2 = 1.0 / dz / dz
+    a = np.array([0.0] + [dzi2 * rho0[k + gw - 1] for k in range(1, nz)])
+    c = np.array([dzi2 * rho0[k + gw] for k in range(nz - 1)] + [0.0])
+    dd = np.array([(rho0[k + gw] + (rho0[k + gw - 1] if k > 0 else 0.0)) if k < nz - 1
+                   else rho0[k + gw - 1] for k in range(nz)]) * dzi2
+    dd[0] = rho0[gw] * dzi2
+    kk0 = kk.copy()
+    kk0[0, 0] = 1.0  # placeholder; the (0,0) mode is zeroed below
+    scr = np.empty((nz,) + kk.shape)
+    x = np.empty_like(spec)
+    b = rho0h[gw] * kk0 - dd[0]
+    scr[0] = c[0] / b
+    x[0] = spec[0] / b
+    for k in range(1, nz):
+        b = rho0h[k + gw] * kk0 - dd[k]
+        m = 1.0 / (b - a[k] * scr[k - 1])
+        scr[k] = c[k] * m
+        x[k] = (spec[k] - a[k] * x[k - 1]) * m
+    for k in range(nz - 2, -1, -1):
+        x[k] = x[k] - scr[k] * x[k + 1]
+    x[:, 0, 0] = 0.0
+    p = np.zeros_like(u)
+    p[sl] = np.real(np.fft.ifft2(x, axes=(1, 2)))
+    fill_z_ghosts(p, gw, 1.0)
+    u = u - (np.roll(p, -1, axis=2) - p) / dx
+    v = v - (np.roll(p, -1, axis=1) - p) / dy
+    w = w.copy()
+    w[:-1] = w[:-1] - (p[1:] - p[:-1]) / dz
+    return fill_z_ghosts(u, gw, 1.0), fill_z_ghosts(v, gw, 1.0), fill_z_ghosts(w, gw, -1.0)
+
+
+def init_dry(Gr, RS, Th, seed=1, sigma=0.5):
+    """Dry state of the 64^3 configuration (isentropic reference, theta_s=300 K, p_s=1000 hPa)."""
+    RS.Pg = 1.0e5
+    RS.Tg = 300.0
+    RS.qtg = 0.0
+    RS.u0 = 0.0
+    RS.v0 = 0.0
+    RS.initialize(Gr, Th)
+    d = Gr.dims
+    gw = d.gw
+    shape = (d.n[2], d.n[1], d.n[0])
+    zh = Gr.z_half[gw:gw + d.n[2]][:, None, None]
+    p0h = RS.p0_half[gw:gw + d.n[2]][:, None, None]
+    theta = 300.0 + np.where(zh < 1000.0, -0.1 + 0.2 * uniform(seed, 20, shape), 0.0)
+    T = theta * H.exner(p0h)
+    s = np.zeros((d.nzg, d.n[1], d.n[0]))
+    s[gw:gw + d.n[2]] = H.entropy_dry(p0h, T)
+    fields = _velocities(Gr, seed, sigma, 5.0)
+    fields["u"], fields["v"], fields["w"] = project_initial(Gr, RS, fields["u"], fields["v"], fields["w"])
+    fields["s"] = fill_z_ghosts(s, gw, 1.0)
+    return fields
+
+
+def moist_profiles(z):
+    thetal = np.where(z <= 520.0, 298.7, 298.7 + (z - 520.0) * 3.85e-3)
+    qt = np.where(z <= 520.0, 17.0 + z * (16.3 - 17.0) / 520.0,
+                  np.where(z <= 2000.0, 16.3 + (z - 520.0) * (4.3 - 16.3) / (2000.0 - 520.0), 4.3))
+    return thetal, qt / 1000.0
+
+
+def init_moist(Gr, RS, Th, seed=2, sigma=0.3):
+    RS.Pg = 1.015e5
+    RS.Tg = 300.4
+    RS.qtg = 0.02245
+    RS.u0 = 0.0
+    RS.v0 = 0.0
+    RS.initialize(Gr, Th)
+    d = Gr.dims
+    gw = d.gw
+    shape = (d.n[2], d.n[1], d.n[0])
+    zh = Gr.z_half[gw:gw + d.n[2]][:, None, None]
+    p0h = RS.p0_half[gw:gw + d.n[2]][:, None, None]
+    thetal, qt = moist_profiles(zh)
+    pert = zh <= 1600.0
+    thetal = thetal + np.where(pert, -0.1 + 0.2 * uniform(seed, 30, shape), 0.0)
+    qt = qt + np.where(pert, (-0.025 + 0.05 * uniform(seed, 31, shape)) / 1000.0, 0.0)
+    T = thetal * H.exner(p0h)
+    s = np.zeros((d.nzg, d.n[1], d.n[0]))
+    q = np.zeros_like(s)
+    s[gw:gw + d.n[2]] = H.entropy_sa(p0h, T, qt, 0.0, 0.0)
+    q[gw:gw + d.n[2]] = qt
+    fields = _velocities(Gr, seed, sigma, -8.75)
+    fields["u"], fields["v"], fields["w"] = project_initial(Gr, RS, fields["u"], fields["v"], fields["w"])
+    fields["s"] = fill_z_ghosts(s, gw, 1.0)
+    fields["qt"] = fill_z_ghosts(q, gw, 1.0)
+    return fields
+
+
+def initialize(namelist, Gr, PV, RS, Th):
+    """Build the initial state and upload it (the InitializationFactory step)."""
+    ini = namelist.get("initialization", {})
+    moist = namelist["microphysics"]["scheme"] == "None_SA"
+    seed = int(ini.get("seed", 1))
+    sigma = float(ini.get("sigma", 0.5))
+    fields = (init_moist if moist else init_dry)(Gr, RS, Th, seed=seed, sigma=sigma)
+    for name, arr in fields.items():
+        if name in ("s", "qt"):
+            PV.set_base_from_device_layout(name, arr)
+        PV.set_device_layout(name, arr)
+    return fields

@@ -1,0 +1,65 @@
+// Entry point of the fused advection passes (see advection_impl.h).
+#include "advection_impl.h"
+
+extern int pycles_check_dims(const pycles_dims *d);
+
+extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                                void *const tend_vel[3], int nscal, const void *const *scal, void *const *tend_scal,
+                                const pycles_scalar_base *base, int scalar_order, int momentum_order, int precision,
+                                void *stream) {
+    int rc = pycles_check_dims(dims);
+    if (rc) return rc;
+    if (!ref || !vel || !vel[0] || !vel[1] || !vel[2]) {
+        pycles_set_error("advection: NULL velocity or profile argument");
+        return PYCLES_ERR_ARG;
+    }
+    if (nscal < 0 || nscal > 2) {
+        pycles_set_error("advection: nscal = %d not in [0, 2]", nscal);
+        return PYCLES_ERR_ARG;
+    }
+    const bool mom = tend_vel != nullptr;
+    if (mom && (!tend_vel[0] || !tend_vel[1] || !tend_vel[2])) {
+        pycles_set_error("advection: NULL velocity tendency");
+        return PYCLES_ERR_ARG;
+    }
+    if (!mom && nscal == 0) return PYCLES_OK;
+    for (int s = 0; s < nscal; ++s)
+        if (!scal || !scal[s] || !tend_scal || !tend_scal[s]) {
+            pycles_set_error("advection: NULL scalar %d", s);
+            return PYCLES_ERR_ARG;
+        }
+    const int so = nscal ? scalar_order : momentum_order;
+    const int mo = mom ? momentum_order : scalar_order;
+    for (int o : {so, mo})
+        if (o != 5 && o != 7 && o != 9) {
+            pycles_set_error("advection: order %d not supported (WENO 5, 7, 9)", o);
+            return PYCLES_ERR_UNSUPPORTED;
+        }
+    const int r = ((so > mo ? so : mo) + 1) / 2;
+    if (dims->gw < r) {
+        pycles_set_error("advection: order %d needs gw >= %d (gw = %lld)", so > mo ? so : mo, r, (long long)dims->gw);
+        return PYCLES_ERR_ARG;
+    }
+    const Geo g = make_geo(dims);
+    if (g.nx < 2 * r || g.ny < 2 * r) {
+        pycles_set_error("advection: nx, ny must be at least %d", 2 * r);
+        return PYCLES_ERR_ARG;
+    }
+    adv::Prof pr{ref->rho0, ref->rho0_half, ref->alpha0, ref->alpha0_half};
+    adv::Fields fl;
+    memset(&fl, 0, sizeof(fl));
+    for (int s = 0; s < nscal; ++s) {
+        fl.q[s] = scal[s];
+        fl.t[s] = tend_scal[s];
+        fl.base[s] = (precision == PYCLES_F32 && base) ? base->base[s] : nullptr;
+    }
+    for (int d = 0; d < 3; ++d) {
+        fl.q[adv::VU + d] = vel[d];
+        fl.t[adv::VU + d] = mom ? tend_vel[d] : nullptr;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == PYCLES_F64) return adv::dispatch<adv::PolicyF64>(g, pr, fl, nscal, mom, so, mo, st);
+    if (precision == PYCLES_F32) return adv::dispatch<adv::PolicyF32>(g, pr, fl, nscal, mom, so, mo, st);
+    pycles_set_error("bad precision %d", precision);
+    return PYCLES_ERR_ARG;
+}
