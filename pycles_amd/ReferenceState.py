@@ -31,7 +31,7 @@ class ReferenceState:
         self.sg = Thermodynamics.entropy(self.Pg, self.Tg, self.qtg, 0.0, 0.0)
 
         def rhs(p, z):
-            T, ql, qi = Thermodynamics.eos(np.exp(p), self.sg, self.qtg)
+            T, ql, qi = Thermodynamics.eos(float(np.exp(p[0])), self.sg, self.qtg)
             return -P.g / (P.Rd * T * (1.0 - self.qtg + P.eps_vi * (self.qtg - ql - qi)))
 
         z = np.array(Gr.z[gw - 1:-gw + 1])

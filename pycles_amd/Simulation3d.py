@@ -84,19 +84,46 @@ class Simulation3d:
                           D.stream())
         self.Th.update(self.Gr, self.Ref, self.PV, self.DV)
 
+    def _mark(self, name):
+        """Record a HIP event on the working stream when timing is enabled."""
+        if self.timers is not None:
+            import torch
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record()
+            self.timers.append((name, ev))
+
     def stage(self, rk_step):
         self.TS.rk_step = rk_step
+        self._mark("thermo")
         self.Th.update(self.Gr, self.Ref, self.PV, self.DV)
+        self._mark("advection")
         if self.fuse_advection:
             advect_fused(self.SA, self.MA, self.Gr, self.Ref, self.PV)
         else:
             self.SA.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa)
             self.MA.update(self.Gr, self.Ref, self.PV, self.Pa)
+        self._mark("rk3")
         self.TS.update(self.Gr, self.PV, self.Pa)
         last = rk_step == self.TS.n_rk_steps - 1
+        self._mark("pressure")
         self.Pr.update(self.Gr, self.Ref, self.DV, self.PV, self.Pa,
                        cfl_out=self.TS.cfl_buffer() if last else None, dt=self.TS.dt)
+        self._mark("end")
         self.TS.adjust_timestep(self.Gr, self.PV, self.DV, self.Pa)
+
+    timers = None
+
+    def phase_times(self):
+        """Milliseconds per phase summed over the recorded stages."""
+        import torch
+        torch.cuda.synchronize()
+        out = {}
+        t = self.timers or []
+        for (name, ev), (_, nxt) in zip(t, t[1:]):
+            if name == "end":
+                continue
+            out[name] = out.get(name, 0.0) + ev.elapsed_time(nxt)
+        return out
 
     def step(self):
         for rk in range(self.TS.n_rk_steps):

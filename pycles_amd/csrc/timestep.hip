@@ -109,6 +109,14 @@ __global__ __launch_bounds__(256) void rk3_kernel(Geo g, int nv, PtrPack pk, int
             val[o] = out;
             if (zero_tend) ten[o] = T(0);
         }
+        if (zero_tend) {
+            // the reference zeroes every tendency, ghost levels included
+            // (the SA buoyancy writes w tendencies on low ghost levels)
+            for (int m = 0; m < g.gw; ++m) {
+                ten[(long long)m * g.plane + col] = T(0);
+                ten[(long long)(g.nzg - 1 - m) * g.plane + col] = T(0);
+            }
+        }
         fill_column_bcs(val, col, g.plane, g.gw, g.nzg, pk.bc[v]);
     }
 }
