@@ -48,8 +48,10 @@ def build_state(cfg_name, nx=None, ny=None, nz=None):
     Gr = Grid(nl)
     RS = ReferenceState(Gr)
     Th = ThermodynamicsFactory(nl)
-    init = cases.init_moist if c["moist"] else cases.init_dry
-    fields = init(Gr, RS, Th, seed=c["seed"], sigma=c["sigma"])
+    if c["moist"]:
+        fields = cases.init_moist(Gr, RS, Th, seed=c["seed"], sigma=c["sigma"], cloud=c.get("cloud"))
+    else:
+        fields = cases.init_dry(Gr, RS, Th, seed=c["seed"], sigma=c["sigma"])
     return c, nl, Gr, RS, Th, fields
 
 

@@ -84,11 +84,15 @@ int pycles_device_count(void);
  * Tendencies are accumulated (tend -= div(flux)), per variable in the
  * reference order x, y, z.  Orders 5, 7, 9 (WENO5/7/9 with centred
  * interpolation of order 4/6/8 for the advecting velocity); gw must be at
- * least (order+1)/2. */
+ * least (order+1)/2.
+ * flags: PYCLES_ADV_TEND_ZERO declares that the scalar, u and v tendencies
+ * are zero on entry (the w tendency is still read: it carries buoyancy); they
+ * are then written instead of read-modified-written, with identical bits. */
+#define PYCLES_ADV_TEND_ZERO 1
 int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
                      void *const tend_vel[3], int nscal, const void *const *scal,
                      void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
-                     int momentum_order, int precision, void *stream);
+                     int momentum_order, int precision, int flags, void *stream);
 
 /* ---- thermodynamics ---------------------------------------------------- */
 /* pv*(T) lookup table (reference Csrc/lookup.h:4-54). */
@@ -108,6 +112,7 @@ typedef struct pycles_sa_options {
     const void *mlp;     /* learned first guess (pycles_mlp handle) or NULL */
     int newton_steps;    /* Newton corrections after the learned guess */
     long long *counters; /* device int64[4] or NULL */
+    int wt_overwrite;    /* 1: the w tendency is zero on entry, write instead of add */
 } pycles_sa_options;
 
 /* Replaces ThermodynamicsSA.update (ThermodynamicsSA.pyx:176-213):
@@ -121,18 +126,22 @@ int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycle
                      const pycles_scalar_base *base, const pycles_sa_options *opt, int precision,
                      void *stream);
 
-/* Replaces ThermodynamicsDry.update (ThermodynamicsDry.pyx:105-120). */
+/* Replaces ThermodynamicsDry.update (ThermodynamicsDry.pyx:105-120).
+ * wt_overwrite as in pycles_sa_options. */
 int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref, void *s, void *wt, void *T,
                       void *alpha, void *buoyancy, void *theta, void *bvf,
-                      const pycles_scalar_base *base, int precision, void *stream);
+                      const pycles_scalar_base *base, int precision, int wt_overwrite, void *stream);
 
 /* ---- time stepping and boundaries -------------------------------------- */
 /* TimeStepping.update_third (TimeStepping.pyx:131-151) for nv fields,
  * followed by the z boundary fill of PrognosticVariables.update_all_bcs
- * (bc_type[v] = 1.0 sym, -1.0 asym).  Tendencies are zeroed. */
+ * (bc_type[v] = 1.0 sym, -1.0 asym).  zero_tendencies = 1 zeroes the
+ * tendencies as the reference does; 0 leaves them for a caller whose next
+ * stage writes them (thermodynamics with wt_overwrite, advection with
+ * PYCLES_ADV_TEND_ZERO) before reading. */
 int pycles_rk3_stage(const pycles_dims *dims, int nv, void *const *values, void *const *tend,
                      void *const *copies, const double *bc_type, int rk_step, double dt,
-                     int precision, void *stream);
+                     int precision, int zero_tendencies, void *stream);
 /* z boundary fill only (set_bcs, prognostic_variables.h:205-270). */
 int pycles_apply_z_bcs(const pycles_dims *dims, int nv, void *const *values,
                        const double *bc_type, int precision, void *stream);

@@ -46,14 +46,14 @@ class ScalarAdvection:
     def initialize(self, Gr, PV, NS=None, Pa=None):
         self.dims = D.dims_of(Gr)
 
-    def update(self, Gr, RS, PV, DV=None, Pa=None):
+    def update(self, Gr, RS, PV, DV=None, Pa=None, tend_zero=False):
         names = _scalar_names(PV)
         if not names:
             return
         ref = D.device_ref(RS)
         _capi.advection(self.dims, ref.c, _vel(PV), None, [PV.ptr(n) for n in names],
                         [PV.tptr(n) for n in names], _base(PV, names), self.order, self.order,
-                        PV.precision, D.stream())
+                        PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
 
 
 class MomentumAdvection:
@@ -68,18 +68,20 @@ class MomentumAdvection:
     def initialize(self, Gr, PV, NS=None, Pa=None):
         self.dims = D.dims_of(Gr)
 
-    def update(self, Gr, RS, PV, Pa=None):
+    def update(self, Gr, RS, PV, Pa=None, tend_zero=False):
         ref = D.device_ref(RS)
         tv = [PV.tptr(PV.velocity_names_directional[d]) for d in range(3)]
         _capi.advection(self.dims, ref.c, _vel(PV), tv, [], [], None, self.order, self.order,
-                        PV.precision, D.stream())
+                        PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
 
 
-def advect_fused(SA: ScalarAdvection, MA: MomentumAdvection, Gr, RS, PV):
-    """SA.update followed by MA.update in one launch sequence."""
+def advect_fused(SA: ScalarAdvection, MA: MomentumAdvection, Gr, RS, PV, tend_zero=False):
+    """SA.update followed by MA.update in one launch sequence.  tend_zero:
+    the scalar, u and v tendencies are zero (or stale and to be replaced) on
+    entry, so they are written rather than accumulated (same bits)."""
     names = _scalar_names(PV)
     ref = D.device_ref(RS)
     tv = [PV.tptr(PV.velocity_names_directional[d]) for d in range(3)]
     _capi.advection(SA.dims, ref.c, _vel(PV), tv, [PV.ptr(n) for n in names],
                     [PV.tptr(n) for n in names], _base(PV, names), SA.order, MA.order,
-                    PV.precision, D.stream())
+                    PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)

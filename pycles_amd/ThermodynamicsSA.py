@@ -62,12 +62,14 @@ class ThermodynamicsSA:
             self.dims = D.dims_of(Gr)
             self.counters = torch.zeros(4, dtype=torch.int64, device=D.require_gpu())
 
-    def options(self):
+    def options(self, wt_overwrite=False):
         return _capi.SAOptions(self.do_qt_clipping, self.max_iter, _capi.LH_VARIABLE,
                                None if self.mlp is None else self.mlp.handle_ptr(),
-                               self.newton_steps, self.counters.data_ptr())
+                               self.newton_steps, self.counters.data_ptr(), wt_overwrite)
 
-    def update(self, Gr, RS, PV, DV):
+    def update(self, Gr, RS, PV, DV, wt_overwrite=False):
+        """wt_overwrite: the w tendency is zero (or stale and to be replaced),
+        so the buoyancy term is written instead of added (same bits)."""
         self._prepare(Gr)
         ref = D.device_ref(RS)
         base = None
@@ -76,7 +78,7 @@ class ThermodynamicsSA:
         _capi.thermo_sa(self.dims, ref.c, self._lt, PV.ptr("s"), PV.ptr("qt"), PV.tptr("w"),
                         DV.ptr("temperature"), DV.ptr("qv"), DV.ptr("ql"), DV.ptr("alpha"),
                         DV.ptr("buoyancy"), DV.ptr("theta_rho"), DV.ptr("buoyancy_frequency"),
-                        DV.ptr("thetali"), base, self.options(), PV.precision, D.stream())
+                        DV.ptr("thetali"), base, self.options(wt_overwrite), PV.precision, D.stream())
 
 
 class ThermodynamicsDry:
@@ -98,14 +100,15 @@ class ThermodynamicsDry:
     def alpha(self, p0, T, qt, qv):
         return float(H.alpha_c(p0, T, 0.0, 0.0))
 
-    def update(self, Gr, RS, PV, DV):
+    def update(self, Gr, RS, PV, DV, wt_overwrite=False):
         if getattr(self, "dims", None) is None:
             self.dims = D.dims_of(Gr)
         ref = D.device_ref(RS)
         base = _capi.ScalarBase(PV.base_ptr("s"), None) if PV.precision == D.F32 else None
         _capi.thermo_dry(self.dims, ref.c, PV.ptr("s"), PV.tptr("w"), DV.ptr("temperature"),
                          DV.ptr("alpha"), DV.ptr("buoyancy"), DV.ptr("theta"),
-                         DV.ptr("buoyancy_frequency"), base, PV.precision, D.stream())
+                         DV.ptr("buoyancy_frequency"), base, PV.precision, D.stream(),
+                         1 if wt_overwrite else 0)
 
 
 def ThermodynamicsFactory(namelist, Micro=None, LH=None, Par=None):

@@ -33,7 +33,7 @@ class TimeStepping:
         self.cfl_dev = torch.zeros(1, dtype=torch.float64, device=D.require_gpu())
         self._cfl_ready = False
 
-    def update(self, Gr, PV, Pa=None):
+    def update(self, Gr, PV, Pa=None, zero_tendencies=True):
         if getattr(self, "copies", None) is None:
             self.copies = torch.zeros_like(PV.values)
             self.dims = D.dims_of(Gr)
@@ -41,7 +41,7 @@ class TimeStepping:
         _capi.rk3_stage(self.dims, [PV.ptr(n) for n in names], [PV.tptr(n) for n in names],
                         [self.copies[PV.get_nv(n)].data_ptr() for n in names],
                         [PV.bc_type[PV.get_nv(n)] for n in names], self.rk_step, self.dt,
-                        PV.precision, D.stream())
+                        PV.precision, D.stream(), 1 if zero_tendencies else 0)
         if self.rk_step == self.n_rk_steps - 1:
             self.t += self.dt
 

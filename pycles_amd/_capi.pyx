@@ -39,6 +39,7 @@ cdef extern from "pycles_amd.h":
         const void *mlp
         int newton_steps
         long long *counters
+        int wt_overwrite
     ctypedef struct pycles_pressure_plan:
         pass
 
@@ -46,6 +47,7 @@ cdef extern from "pycles_amd.h":
     int PYCLES_F32
     int PYCLES_LH_VARIABLE
     int PYCLES_LH_CONSTANT
+    int PYCLES_ADV_TEND_ZERO
 
     const char *pycles_last_error()
     const char *pycles_version()
@@ -53,7 +55,7 @@ cdef extern from "pycles_amd.h":
     int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const *vel,
                          void *const *tend_vel, int nscal, const void *const *scal,
                          void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
-                         int momentum_order, int precision, void *stream)
+                         int momentum_order, int precision, int flags, void *stream)
     int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt,
                          void *s, void *qt, void *wt, void *T, void *qv, void *ql, void *alpha,
                          void *buoyancy, void *theta_rho, void *bvf, void *thetali,
@@ -61,10 +63,10 @@ cdef extern from "pycles_amd.h":
                          void *stream)
     int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref, void *s, void *wt, void *T,
                           void *alpha, void *buoyancy, void *theta, void *bvf,
-                          const pycles_scalar_base *base, int precision, void *stream)
+                          const pycles_scalar_base *base, int precision, int wt_overwrite, void *stream)
     int pycles_rk3_stage(const pycles_dims *dims, int nv, void *const *values, void *const *tend,
                          void *const *copies, const double *bc_type, int rk_step, double dt,
-                         int precision, void *stream)
+                         int precision, int zero_tendencies, void *stream)
     int pycles_apply_z_bcs(const pycles_dims *dims, int nv, void *const *values,
                            const double *bc_type, int precision, void *stream)
     int pycles_level_mean(const pycles_dims *dims, const void *field, double *mean, int precision,
@@ -88,6 +90,7 @@ F64 = PYCLES_F64
 F32 = PYCLES_F32
 LH_VARIABLE = PYCLES_LH_VARIABLE
 LH_CONSTANT = PYCLES_LH_CONSTANT
+ADV_TEND_ZERO = PYCLES_ADV_TEND_ZERO
 
 
 class PyclesError(RuntimeError):
@@ -159,13 +162,14 @@ cdef class SAOptions:
     cdef pycles_sa_options o
 
     def __init__(self, do_qt_clipping=True, max_iter=100, latent_heat=0, mlp=None,
-                 newton_steps=0, counters=None):
+                 newton_steps=0, counters=None, wt_overwrite=False):
         self.o.do_qt_clipping = 1 if do_qt_clipping else 0
         self.o.max_iter = max_iter
         self.o.latent_heat = latent_heat
         self.o.mlp = _p(mlp)
         self.o.newton_steps = newton_steps
         self.o.counters = <long long *> _p(counters)
+        self.o.wt_overwrite = 1 if wt_overwrite else 0
 
 
 cdef class PressurePlan:
@@ -190,7 +194,7 @@ cdef class PressurePlan:
 
 
 def advection(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase base,
-              int scalar_order, int momentum_order, int precision, stream):
+              int scalar_order, int momentum_order, int precision, stream, int flags=0):
     cdef const void *v3[3]
     cdef void *t3[3]
     cdef const void *sc[2]
@@ -204,7 +208,7 @@ def advection(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase bas
         ts[i] = _p(tend_scal[i])
     _check(pycles_advection(&dims.d, &ref.r, v3, t3 if tend_vel is not None else NULL, ns, sc, ts,
                             &base.b if base is not None else NULL, scalar_order, momentum_order,
-                            precision, _p(stream)))
+                            precision, flags, _p(stream)))
 
 
 def thermo_sa(Dims dims, Ref ref, Lookup lt, s, qt, wt, T, qv, ql, alpha, buoyancy, theta_rho,
@@ -215,13 +219,14 @@ def thermo_sa(Dims dims, Ref ref, Lookup lt, s, qt, wt, T, qv, ql, alpha, buoyan
 
 
 def thermo_dry(Dims dims, Ref ref, s, wt, T, alpha, buoyancy, theta, bvf, ScalarBase base,
-               int precision, stream):
+               int precision, stream, int wt_overwrite=0):
     _check(pycles_thermo_dry(&dims.d, &ref.r, _p(s), _p(wt), _p(T), _p(alpha), _p(buoyancy),
                              _p(theta), _p(bvf), &base.b if base is not None else NULL, precision,
-                             _p(stream)))
+                             wt_overwrite, _p(stream)))
 
 
-def rk3_stage(Dims dims, values, tend, copies, bc_type, int rk_step, double dt, int precision, stream):
+def rk3_stage(Dims dims, values, tend, copies, bc_type, int rk_step, double dt, int precision, stream,
+              int zero_tendencies=1):
     cdef int nv = len(values)
     cdef void *v[8]
     cdef void *t[8]
@@ -234,7 +239,7 @@ def rk3_stage(Dims dims, values, tend, copies, bc_type, int rk_step, double dt, 
         t[i] = _p(tend[i])
         c[i] = _p(copies[i])
         bc[i] = bc_type[i]
-    _check(pycles_rk3_stage(&dims.d, nv, v, t, c, bc, rk_step, dt, precision, _p(stream)))
+    _check(pycles_rk3_stage(&dims.d, nv, v, t, c, bc, rk_step, dt, precision, zero_tendencies, _p(stream)))
 
 
 def apply_z_bcs(Dims dims, values, bc_type, int precision, stream):

@@ -18,7 +18,10 @@ Configurations (sizes, spacings, profiles) follow the benchmark definitions:
   moist     BOMEX-like moist profiles (qt 17 -> 16.3 g/kg at 520 m -> 4.3 g/kg
             at 2 km, held above; theta_l 298.7 K to 520 m then +3.85 K/km;
             qt +-0.025 g/kg, theta_l +-0.1 K uniform noise below 1.6 km;
-            u=-8.75+N(0,sigma), v,w=N(0,sigma)) on any grid.
+            u=-8.75+N(0,sigma), v,w=N(0,sigma)) on any grid.  These profiles
+            peak at 94% relative humidity (510 m), so the configurations that
+            need saturated cells (the 512^2 and 1024^2 ones: ~20% of cells)
+            add a saturated layer, qt = max(qt, 1.02 qv*), for 500-1500 m.
 Profiles are evaluated at cell centres z_half (the reference's InitBomex
 evaluates them at ``Gr.zl_half``, which Grid.pyx:166-167 sets to full levels).
 """
@@ -79,9 +82,9 @@ CONFIGS = {
     "moist256": dict(nx=256, ny=256, nz=128, gw=3, dx=50.0, dy=50.0, dz=20.0, moist=True,
                      order=(5, 5), dt=2.0, steps=10, seed=2, sigma=0.3),
     "moist512": dict(nx=512, ny=512, nz=256, gw=3, dx=50.0, dy=50.0, dz=20.0, moist=True,
-                     order=(5, 5), dt=1.0, steps=100, seed=3, sigma=0.3),
+                     order=(5, 5), dt=1.0, steps=100, seed=3, sigma=0.3, cloud=(500.0, 1500.0)),
     "moist1024": dict(nx=1024, ny=1024, nz=192, gw=5, dx=25.0, dy=25.0, dz=10.0, moist=True,
-                      order=(9, 5), dt=0.5, steps=20, seed=5, sigma=1.0),
+                      order=(9, 5), dt=0.5, steps=20, seed=5, sigma=1.0, cloud=(500.0, 1500.0)),
 }
 
 
@@ -97,7 +100,7 @@ def namelist(nx, ny, nz, gw, dx, dy, dz, moist, order=(5, 5), dt=1.0, t_max=1e9,
         "microphysics": {"scheme": "None_SA" if moist else "None_Dry"},
         "mi355x": {"precision": precision},
         "initialization": {"seed": extra.get("seed", 1), "sigma": extra.get("sigma", 0.5),
-                           "case": "moist" if moist else "dry"},
+                           "case": "moist" if moist else "dry", "cloud": extra.get("cloud")},
     }
     return nl
 
@@ -210,7 +213,7 @@ def moist_profiles(z):
     return thetal, qt / 1000.0
 
 
-def init_moist(Gr, RS, Th, seed=2, sigma=0.3):
+def init_moist(Gr, RS, Th, seed=2, sigma=0.3, cloud=None):
     RS.Pg = 1.015e5
     RS.Tg = 300.4
     RS.qtg = 0.02245
@@ -223,6 +226,12 @@ def init_moist(Gr, RS, Th, seed=2, sigma=0.3):
     zh = Gr.z_half[gw:gw + d.n[2]][:, None, None]
     p0h = RS.p0_half[gw:gw + d.n[2]][:, None, None]
     thetal, qt = moist_profiles(zh)
+    if cloud is not None:
+        # saturated layer: qt raised to 1.02 qv*(T, p0) between cloud[0] and cloud[1]
+        T0 = thetal * H.exner(p0h)
+        qvs = H.qv_star_c(p0h, qt, Th.CC.table(T0))
+        lay = (zh >= cloud[0]) & (zh <= cloud[1])
+        qt = np.where(lay, np.maximum(qt, 1.02 * qvs), qt)
     pert = zh <= 1600.0
     thetal = thetal + np.where(pert, -0.1 + 0.2 * uniform(seed, 30, shape), 0.0)
     qt = qt + np.where(pert, (-0.025 + 0.05 * uniform(seed, 31, shape)) / 1000.0, 0.0)
@@ -244,7 +253,10 @@ def initialize(namelist, Gr, PV, RS, Th):
     moist = namelist["microphysics"]["scheme"] == "None_SA"
     seed = int(ini.get("seed", 1))
     sigma = float(ini.get("sigma", 0.5))
-    fields = (init_moist if moist else init_dry)(Gr, RS, Th, seed=seed, sigma=sigma)
+    if moist:
+        fields = init_moist(Gr, RS, Th, seed=seed, sigma=sigma, cloud=ini.get("cloud"))
+    else:
+        fields = init_dry(Gr, RS, Th, seed=seed, sigma=sigma)
     for name, arr in fields.items():
         if name in ("s", "qt"):
             PV.set_base_from_device_layout(name, arr)

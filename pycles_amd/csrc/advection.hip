@@ -2,11 +2,15 @@
 #include "advection_impl.h"
 
 extern int pycles_check_dims(const pycles_dims *d);
+int adv_dispatch_f64(const Geo &g, const adv::Prof &pr, const adv::Fields &fl, int nsc, bool mom, int so, int mo,
+                     hipStream_t st);
+int adv_dispatch_f32(const Geo &g, const adv::Prof &pr, const adv::Fields &fl, int nsc, bool mom, int so, int mo,
+                     hipStream_t st);
 
 extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
                                 void *const tend_vel[3], int nscal, const void *const *scal, void *const *tend_scal,
                                 const pycles_scalar_base *base, int scalar_order, int momentum_order, int precision,
-                                void *stream) {
+                                int flags, void *stream) {
     int rc = pycles_check_dims(dims);
     if (rc) return rc;
     if (!ref || !vel || !vel[0] || !vel[1] || !vel[2]) {
@@ -48,6 +52,7 @@ extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, 
     adv::Prof pr{ref->rho0, ref->rho0_half, ref->alpha0, ref->alpha0_half};
     adv::Fields fl;
     memset(&fl, 0, sizeof(fl));
+    fl.tzero = (flags & PYCLES_ADV_TEND_ZERO) ? 1 : 0;
     for (int s = 0; s < nscal; ++s) {
         fl.q[s] = scal[s];
         fl.t[s] = tend_scal[s];
@@ -58,8 +63,13 @@ extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, 
         fl.t[adv::VU + d] = mom ? tend_vel[d] : nullptr;
     }
     hipStream_t st = (hipStream_t)stream;
-    if (precision == PYCLES_F64) return adv::dispatch<adv::PolicyF64>(g, pr, fl, nscal, mom, so, mo, st);
-    if (precision == PYCLES_F32) return adv::dispatch<adv::PolicyF32>(g, pr, fl, nscal, mom, so, mo, st);
+    const size_t esz = precision == PYCLES_F64 ? 8 : 4;
+    if ((size_t)g.plane * g.nzg * esz >= ((size_t)1 << 32)) {
+        pycles_set_error("advection: one field exceeds 4 GiB (32-bit offsets)");
+        return PYCLES_ERR_UNSUPPORTED;
+    }
+    if (precision == PYCLES_F64) return adv_dispatch_f64(g, pr, fl, nscal, mom, so, mo, st);
+    if (precision == PYCLES_F32) return adv_dispatch_f32(g, pr, fl, nscal, mom, so, mo, st);
     pycles_set_error("bad precision %d", precision);
     return PYCLES_ERR_ARG;
 }

@@ -2,7 +2,8 @@
 // scalars and the nine momentum (advected, advecting) pairs, fused per
 // direction.  Included by advection_f64.hip and advection_f32.hip, each of
 // which instantiates one precision policy with its own floating-point
-// contraction setting.
+// contraction setting (off for FP64, which reproduces the reference's
+// rounding; on for FP32).
 //
 // Reference semantics (all cited files under the reference tree):
 //   scalar flux      scalar_advection.h:319-578, rho0_half in x/y, rho0 in z
@@ -16,24 +17,43 @@
 // A face flux F[c] sits between cells c and c+1.  Each direction's pass
 // computes every face flux exactly once: the y and z passes march along the
 // line and keep the previous face flux in registers, the x pass holds a whole
-// periodic row in LDS.
+// periodic row in LDS.  Field offsets are 32-bit byte offsets from the field
+// base (one field must stay below 4 GiB), so loads use a scalar base plus a
+// 32-bit vector offset; wrapped horizontal neighbour offsets are computed
+// once per thread, outside the march.
 #pragma once
 #include "common.h"
 
 namespace adv {
 
-// Per-variable slot numbering inside the fused kernels.
 enum { VS0 = 0, VS1 = 1, VU = 2, VV = 3, VW = 4 };
 
 struct Fields {
     const void *q[5];  // s0, s1, u, v, w
     void *t[5];        // tendencies
     const double *base[2];
+    int tzero;  // scalar, u, v tendencies are zero on entry (x pass writes them)
 };
 
 struct Prof {
     const double *rho0, *rho0_half, *alpha0, *alpha0_half;
 };
+
+template <typename T>
+__device__ __forceinline__ T ld(const T *p, unsigned idx) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ T *at(T *p, unsigned idx) {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+
+// tend -= d, or tend = 0 - d when the tendency is known to be zero (same bits).
+template <typename TS>
+__device__ __forceinline__ void sub_into(TS *p, unsigned o, TS d, bool zero) {
+    if (zero) *at(p, o) = TS(0) - d;
+    else *at(p, o) -= d;
+}
 
 // -------------------------------------------------------------------------
 // FP64 policy: storage and arithmetic in double, reference operation order.
@@ -42,8 +62,7 @@ struct PolicyF64 {
     using TC = double;
     using SFlux = double;
     static constexpr bool kBase = false;
-    static __device__ __forceinline__ TC load(const TS *p, long long i) { return p[i]; }
-    template <int O>
+    template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *, TC U, double rho) {
         return upwind_flux<O, double>(win, U, rho);
     }
@@ -58,18 +77,18 @@ struct PolicyF64 {
     static __device__ __forceinline__ TC mom_div(TC Fc, TC Fp, double alpha, double dxi) {
         return alpha * (Fc - Fp) * dxi;
     }
-    static __device__ __forceinline__ void acc(TS *p, long long i, TC d) { p[i] -= d; }
 };
 
 // -------------------------------------------------------------------------
 // FP32 policy.  Scalars are stored as deviations phi' = phi - base(k) from an
 // FP64 level profile.  The reconstruction is evaluated on offsets from the
-// upwind centre cell, d_m = phi_m - phi_ctr (exact for neighbouring values;
-// WENO is invariant to adding a constant), so the large constant part never
-// enters FP32 arithmetic.  The face flux is carried as a pair: the small part
-// rho*U*(phi'_ctr + R(d)) in FP32 and the base part rho*U*base_ctr in FP64,
-// and the flux difference of the base part is formed in FP64 before the
-// tendency is rounded to FP32 (compensated flux divergence).
+// upwind centre cell, d_m = phi_m - phi_ctr (WENO is invariant to adding a
+// constant, and neighbouring values subtract without rounding), so the large
+// constant part never enters FP32 arithmetic.  The face flux is carried as a
+// pair: the small part rho*U*(phi'_ctr + R(d)) in FP32 and the base part
+// rho*U*base_ctr in FP64, and the base part's flux difference is formed in
+// FP64 before the tendency is rounded to FP32 (compensated flux divergence).
+// BV: the base varies along the line (z); in x and y it is one value.
 struct SF32 {
     float f;
     double b;
@@ -79,19 +98,25 @@ struct PolicyF32 {
     using TC = float;
     using SFlux = SF32;
     static constexpr bool kBase = true;
-    static __device__ __forceinline__ TC load(const TS *p, long long i) { return p[i]; }
-    template <int O>
+    template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *bwin, TC U, double rho) {
         constexpr int R = Interp<O, float>::R;
         const bool pos = U >= 0.0f;
         const int ctr = pos ? R - 1 : R;
         const float pc = win[ctr];
-        const double bc = bwin[ctr];
         float q[2 * R - 1];
+        double bc;
+        if (BV) {
+            bc = pos ? bwin[R - 1] : bwin[R];
 #pragma unroll
-        for (int m = 0; m < 2 * R - 1; ++m) {
-            const int src = pos ? m : 2 * R - 1 - m;
-            q[m] = (win[src] - pc) + (float)(bwin[src] - bc);
+            for (int m = 0; m < 2 * R - 1; ++m) {
+                const int src = pos ? m : 2 * R - 1 - m;
+                q[m] = (win[src] - pc) + (float)(bwin[src] - bc);
+            }
+        } else {
+            bc = bwin[0];
+#pragma unroll
+            for (int m = 0; m < 2 * R - 1; ++m) q[m] = (pos ? win[m] : win[2 * R - 1 - m]) - pc;
         }
         const float r = pc + Interp<O, float>::weno(q);
         SF32 F;
@@ -118,36 +143,16 @@ struct PolicyF32 {
     static __device__ __forceinline__ TC mom_div(TC Fc, TC Fp, double alpha, double dxi) {
         return (float)alpha * (Fc - Fp) * (float)dxi;
     }
-    static __device__ __forceinline__ void acc(TS *p, long long i, TC d) { p[i] -= d; }
 };
 
-// Load the centred advecting-velocity stencil (order MO) along a periodic
-// horizontal direction from global memory.
+// Centred interpolation (order MO) from NC = 2(R-1) loads at the given offsets.
 template <class P, int MO>
-__device__ __forceinline__ typename P::TC centred_x(const typename P::TS *f, const Geo &g, int i, int j, int k) {
+__device__ __forceinline__ typename P::TC centred_at(const typename P::TS *f, const unsigned *off) {
     using TC = typename P::TC;
-    constexpr int RM = Interp<MO, TC>::R;
-    TC q[2 * (RM - 1)];
+    constexpr int NC = 2 * (Interp<MO, TC>::R - 1);
+    TC q[NC];
 #pragma unroll
-    for (int m = 0; m < 2 * (RM - 1); ++m) q[m] = P::load(f, gidx(g, wrapi(i + m - (RM - 2), g.nx), j, k));
-    return Interp<MO, TC>::centred(q);
-}
-template <class P, int MO>
-__device__ __forceinline__ typename P::TC centred_y(const typename P::TS *f, const Geo &g, int i, int j, int k) {
-    using TC = typename P::TC;
-    constexpr int RM = Interp<MO, TC>::R;
-    TC q[2 * (RM - 1)];
-#pragma unroll
-    for (int m = 0; m < 2 * (RM - 1); ++m) q[m] = P::load(f, gidx(g, i, wrapi(j + m - (RM - 2), g.ny), k));
-    return Interp<MO, TC>::centred(q);
-}
-template <class P, int MO>
-__device__ __forceinline__ typename P::TC centred_z(const typename P::TS *f, const Geo &g, int i, int j, int k) {
-    using TC = typename P::TC;
-    constexpr int RM = Interp<MO, TC>::R;
-    TC q[2 * (RM - 1)];
-#pragma unroll
-    for (int m = 0; m < 2 * (RM - 1); ++m) q[m] = P::load(f, gidx(g, i, j, k + m - (RM - 2)));
+    for (int m = 0; m < NC; ++m) q[m] = (TC)ld(f, off[m]);
     return Interp<MO, TC>::centred(q);
 }
 
@@ -161,40 +166,45 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     using SFlux = typename P::SFlux;
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
+    constexpr int NC = 2 * (RM - 1);
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     if (i >= g.nx || j >= g.ny) return;  // no barriers in this kernel
     const int k0 = g.gw + blockIdx.z * kchunk;
     const int k1 = min(k0 + kchunk, g.nz + g.gw);
-    const long long col = (long long)j * g.nx + i;
-    const TS *ps[2] = {(const TS *)fl.q[0], (const TS *)fl.q[1]};
+    const unsigned plane = (unsigned)g.plane;
+    const unsigned col = (unsigned)j * g.nx + i;
+    const TS *ps0 = (const TS *)fl.q[0];
+    const TS *ps1 = (const TS *)fl.q[1];
     const TS *pu = (const TS *)fl.q[VU];
     const TS *pv = (const TS *)fl.q[VV];
     const TS *pw = (const TS *)fl.q[VW];
-
+    // horizontal neighbour column offsets for the centred advecting w
+    unsigned cx[NC], cy[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+        cx[m] = (unsigned)j * g.nx + wrapi(i + m - (RM - 2), g.nx);
+        cy[m] = (unsigned)wrapi(j + m - (RM - 2), g.ny) * g.nx + i;
+    }
     TC ws[NSC > 0 ? NSC : 1][2 * RS];
     double bw[NSC > 0 ? NSC : 1][2 * RS];
-    TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];  // momentum windows
-    TC wwS[2 * RS];                               // w at scalar-window levels (only [RS-1] used)
-    (void)wwS;
-
+    TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
     int c = k0 - 1;
-    // initial windows: levels c-R+1 .. c+R
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc)
 #pragma unroll
         for (int m = 0; m < 2 * RS; ++m) {
             const int lev = c - (RS - 1) + m;
-            ws[sc][m] = P::load(ps[sc], lev * g.plane + col);
+            ws[sc][m] = (TC)ld(sc ? ps1 : ps0, lev * plane + col);
             bw[sc][m] = P::kBase ? (fl.base[sc] ? fl.base[sc][lev] : 0.0) : 0.0;
         }
     if (MOM) {
 #pragma unroll
         for (int m = 0; m < 2 * RM; ++m) {
-            const long long o = (long long)(c - (RM - 1) + m) * g.plane + col;
-            wsu[m] = P::load(pu, o);
-            wsv[m] = P::load(pv, o);
-            wsw[m] = P::load(pw, o);
+            const unsigned o = (unsigned)(c - (RM - 1) + m) * plane + col;
+            wsu[m] = (TC)ld(pu, o);
+            wsv[m] = (TC)ld(pv, o);
+            wsw[m] = (TC)ld(pw, o);
         }
     }
     SFlux Fs_prev[NSC > 0 ? NSC : 1];
@@ -202,35 +212,39 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     const double dzi = g.dxi[2];
 
     for (; c < k1; ++c) {
+        const unsigned lo = (unsigned)c * plane;
         const double rho0c = pr.rho0[c];
-        const TC W = P::load(pw, c * g.plane + col);
+        const TC W = MOM ? wsw[RM - 1] : (TC)ld(pw, lo + col);
         SFlux Fs[NSC > 0 ? NSC : 1];
 #pragma unroll
-        for (int sc = 0; sc < NSC; ++sc) Fs[sc] = P::template scal_flux<SO>(ws[sc], bw[sc], W, rho0c);
+        for (int sc = 0; sc < NSC; ++sc) Fs[sc] = P::template scal_flux<SO, true>(ws[sc], bw[sc], W, rho0c);
         TC Fu = 0, Fv = 0, Fw = 0;
         if (MOM) {
-            // u advected by w: U = centred(w along x), rho0[k]
-            const TC Uu = centred_x<P, MO>(pw, g, i, j, c);
-            Fu = P::template mom_flux<MO>(wsu, Uu, rho0c);
-            // v advected by w: U = centred(w along y), rho0[k]
-            const TC Uv = centred_y<P, MO>(pw, g, i, j, c);
-            Fv = P::template mom_flux<MO>(wsv, Uv, rho0c);
-            // w advected by w: U = centred(w along z) from the window, rho0_half[k+1]
-            TC q[2 * (RM - 1)];
+            unsigned ox[NC], oy[NC];
 #pragma unroll
-            for (int m = 0; m < 2 * (RM - 1); ++m) q[m] = wsw[m + 1];
-            const TC Uw = Interp<MO, TC>::centred(q);
-            Fw = P::template mom_flux<MO>(wsw, Uw, pr.rho0_half[c + 1]);
+            for (int m = 0; m < NC; ++m) {
+                ox[m] = lo + cx[m];
+                oy[m] = lo + cy[m];
+            }
+            // u advected by w: U = centred(w along x), rho0[k]
+            Fu = P::template mom_flux<MO>(wsu, centred_at<P, MO>(pw, ox), rho0c);
+            // v advected by w: U = centred(w along y), rho0[k]
+            Fv = P::template mom_flux<MO>(wsv, centred_at<P, MO>(pw, oy), rho0c);
+            // w advected by w: U = centred(w along z) from the window, rho0_half[k+1]
+            TC q[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) q[m] = wsw[m + 1];
+            Fw = P::template mom_flux<MO>(wsw, Interp<MO, TC>::centred(q), pr.rho0_half[c + 1]);
         }
         if (c >= k0) {
-            const long long o = c * g.plane + col;
+            const unsigned o = lo + col;
             const double ah = pr.alpha0_half[c];
 #pragma unroll
-            for (int sc = 0; sc < NSC; ++sc) P::acc((TS *)fl.t[sc], o, P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi));
+            for (int sc = 0; sc < NSC; ++sc) *at((TS *)fl.t[sc], o) -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi);
             if (MOM) {
-                P::acc((TS *)fl.t[VU], o, P::mom_div(Fu, Fu_prev, ah, dzi));
-                P::acc((TS *)fl.t[VV], o, P::mom_div(Fv, Fv_prev, ah, dzi));
-                P::acc((TS *)fl.t[VW], o, P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
+                *at((TS *)fl.t[VU], o) -= P::mom_div(Fu, Fu_prev, ah, dzi);
+                *at((TS *)fl.t[VV], o) -= P::mom_div(Fv, Fv_prev, ah, dzi);
+                *at((TS *)fl.t[VW], o) -= P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi);
             }
         }
 #pragma unroll
@@ -239,7 +253,6 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         Fv_prev = Fv;
         Fw_prev = Fw;
         if (c + 1 < k1) {
-            // shift windows and load the next level
 #pragma unroll
             for (int sc = 0; sc < NSC; ++sc) {
 #pragma unroll
@@ -248,7 +261,7 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
                     bw[sc][m] = bw[sc][m + 1];
                 }
                 const int lev = c + 1 + RS;
-                ws[sc][2 * RS - 1] = P::load(ps[sc], lev * g.plane + col);
+                ws[sc][2 * RS - 1] = (TC)ld(sc ? ps1 : ps0, lev * plane + col);
                 bw[sc][2 * RS - 1] = P::kBase ? (fl.base[sc] ? fl.base[sc][lev] : 0.0) : 0.0;
             }
             if (MOM) {
@@ -258,10 +271,10 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
                     wsv[m] = wsv[m + 1];
                     wsw[m] = wsw[m + 1];
                 }
-                const long long o = (long long)(c + 1 + RM) * g.plane + col;
-                wsu[2 * RM - 1] = P::load(pu, o);
-                wsv[2 * RM - 1] = P::load(pv, o);
-                wsw[2 * RM - 1] = P::load(pw, o);
+                const unsigned o = (unsigned)(c + 1 + RM) * plane + col;
+                wsu[2 * RM - 1] = (TC)ld(pu, o);
+                wsv[2 * RM - 1] = (TC)ld(pv, o);
+                wsw[2 * RM - 1] = (TC)ld(pw, o);
             }
         }
     }
@@ -277,14 +290,18 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     using SFlux = typename P::SFlux;
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
+    constexpr int NC = 2 * (RM - 1);
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int kk = blockIdx.y * 4 + threadIdx.y;
     if (i >= g.nx || kk >= g.nz) return;
     const int k = kk + g.gw;
     const int j0 = blockIdx.z * jchunk;
     const int j1 = min(j0 + jchunk, g.ny);
-    const long long lev = (long long)k * g.plane;
-    const TS *ps[2] = {(const TS *)fl.q[0], (const TS *)fl.q[1]};
+    const unsigned plane = (unsigned)g.plane;
+    const unsigned nx = (unsigned)g.nx;
+    const unsigned lev = (unsigned)k * plane + i;  // offset of (i, 0, k)
+    const TS *ps0 = (const TS *)fl.q[0];
+    const TS *ps1 = (const TS *)fl.q[1];
     const TS *pu = (const TS *)fl.q[VU];
     const TS *pv = (const TS *)fl.q[VV];
     const TS *pw = (const TS *)fl.q[VW];
@@ -293,59 +310,67 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     const double ah = pr.alpha0_half[k];
     const double a0 = pr.alpha0[k];
     const double dyi = g.dxi[1];
-
+    // offsets (relative to row start) of the centred stencils of v along x and z
+    int dxo[NC];
+    int dzo[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+        dxo[m] = wrapi(i + m - (RM - 2), g.nx) - i;
+        dzo[m] = (m - (RM - 2)) * (int)plane;
+    }
+    double bw[NSC > 0 ? NSC : 1][1];
+#pragma unroll
+    for (int sc = 0; sc < NSC; ++sc) bw[sc][0] = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
     TC ws[NSC > 0 ? NSC : 1][2 * RS];
-    double bw[NSC > 0 ? NSC : 1][2 * RS];
     TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
     int c = j0 - 1;
 #pragma unroll
-    for (int sc = 0; sc < NSC; ++sc) {
-        const double b = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
+    for (int sc = 0; sc < NSC; ++sc)
 #pragma unroll
-        for (int m = 0; m < 2 * RS; ++m) {
-            ws[sc][m] = P::load(ps[sc], lev + (long long)wrapi(c - (RS - 1) + m, g.ny) * g.nx + i);
-            bw[sc][m] = b;
-        }
-    }
+        for (int m = 0; m < 2 * RS; ++m)
+            ws[sc][m] = (TC)ld(sc ? ps1 : ps0, lev + (unsigned)wrapi(c - (RS - 1) + m, g.ny) * nx);
     if (MOM) {
 #pragma unroll
         for (int m = 0; m < 2 * RM; ++m) {
-            const long long o = lev + (long long)wrapi(c - (RM - 1) + m, g.ny) * g.nx + i;
-            wsu[m] = P::load(pu, o);
-            wsv[m] = P::load(pv, o);
-            wsw[m] = P::load(pw, o);
+            const unsigned o = lev + (unsigned)wrapi(c - (RM - 1) + m, g.ny) * nx;
+            wsu[m] = (TC)ld(pu, o);
+            wsv[m] = (TC)ld(pv, o);
+            wsw[m] = (TC)ld(pw, o);
         }
     }
+    // next rows to load (wrapped) for the scalar and momentum windows
+    int jn_s = wrapi(c + RS, g.ny), jn_m = wrapi(c + RM, g.ny);
+    int cw = c < 0 ? c + g.ny : c;
     SFlux Fs_prev[NSC > 0 ? NSC : 1];
     TC Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
     for (; c < j1; ++c) {
-        const int cw = wrapi(c, g.ny);
+        const unsigned orow = lev + (unsigned)cw * nx;  // (i, c, k)
         SFlux Fs[NSC > 0 ? NSC : 1];
-        TC V;
-        if (MOM) V = wsv[RM - 1];
-        else V = P::load(pv, lev + (long long)cw * g.nx + i);
+        const TC V = MOM ? wsv[RM - 1] : (TC)ld(pv, orow);
 #pragma unroll
-        for (int sc = 0; sc < NSC; ++sc) Fs[sc] = P::template scal_flux<SO>(ws[sc], bw[sc], V, rh);
+        for (int sc = 0; sc < NSC; ++sc) Fs[sc] = P::template scal_flux<SO, false>(ws[sc], bw[sc], V, rh);
         TC Fu = 0, Fv = 0, Fw = 0;
         if (MOM) {
-            const TC Uu = centred_x<P, MO>(pv, g, i, cw, k);  // v along x
-            Fu = P::template mom_flux<MO>(wsu, Uu, rh);
-            TC q[2 * (RM - 1)];
+            unsigned ox[NC], oz[NC];
 #pragma unroll
-            for (int m = 0; m < 2 * (RM - 1); ++m) q[m] = wsv[m + 1];  // v along y
-            const TC Uv = Interp<MO, TC>::centred(q);
-            Fv = P::template mom_flux<MO>(wsv, Uv, rh);
-            const TC Uw = centred_z<P, MO>(pv, g, i, cw, k);  // v along z
-            Fw = P::template mom_flux<MO>(wsw, Uw, r0);
+            for (int m = 0; m < NC; ++m) {
+                ox[m] = orow + dxo[m];
+                oz[m] = orow + dzo[m];
+            }
+            Fu = P::template mom_flux<MO>(wsu, centred_at<P, MO>(pv, ox), rh);  // v along x
+            TC q[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) q[m] = wsv[m + 1];  // v along y
+            Fv = P::template mom_flux<MO>(wsv, Interp<MO, TC>::centred(q), rh);
+            Fw = P::template mom_flux<MO>(wsw, centred_at<P, MO>(pv, oz), r0);  // v along z
         }
         if (c >= j0) {
-            const long long o = lev + (long long)c * g.nx + i;
 #pragma unroll
-            for (int sc = 0; sc < NSC; ++sc) P::acc((TS *)fl.t[sc], o, P::scal_div(Fs[sc], Fs_prev[sc], ah, dyi));
+            for (int sc = 0; sc < NSC; ++sc) *at((TS *)fl.t[sc], orow) -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dyi);
             if (MOM) {
-                P::acc((TS *)fl.t[VU], o, P::mom_div(Fu, Fu_prev, ah, dyi));
-                P::acc((TS *)fl.t[VV], o, P::mom_div(Fv, Fv_prev, ah, dyi));
-                P::acc((TS *)fl.t[VW], o, P::mom_div(Fw, Fw_prev, a0, dyi));
+                *at((TS *)fl.t[VU], orow) -= P::mom_div(Fu, Fu_prev, ah, dyi);
+                *at((TS *)fl.t[VV], orow) -= P::mom_div(Fv, Fv_prev, ah, dyi);
+                *at((TS *)fl.t[VW], orow) -= P::mom_div(Fw, Fw_prev, a0, dyi);
             }
         }
 #pragma unroll
@@ -353,12 +378,15 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
         Fu_prev = Fu;
         Fv_prev = Fv;
         Fw_prev = Fw;
+        cw = cw + 1 == g.ny ? 0 : cw + 1;
+        jn_s = jn_s + 1 == g.ny ? 0 : jn_s + 1;
+        jn_m = jn_m + 1 == g.ny ? 0 : jn_m + 1;
         if (c + 1 < j1) {
 #pragma unroll
             for (int sc = 0; sc < NSC; ++sc) {
 #pragma unroll
                 for (int m = 0; m < 2 * RS - 1; ++m) ws[sc][m] = ws[sc][m + 1];
-                ws[sc][2 * RS - 1] = P::load(ps[sc], lev + (long long)wrapi(c + 1 + RS, g.ny) * g.nx + i);
+                ws[sc][2 * RS - 1] = (TC)ld(sc ? ps1 : ps0, lev + (unsigned)jn_s * nx);
             }
             if (MOM) {
 #pragma unroll
@@ -367,10 +395,10 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
                     wsv[m] = wsv[m + 1];
                     wsw[m] = wsw[m + 1];
                 }
-                const long long o = lev + (long long)wrapi(c + 1 + RM, g.ny) * g.nx + i;
-                wsu[2 * RM - 1] = P::load(pu, o);
-                wsv[2 * RM - 1] = P::load(pv, o);
-                wsw[2 * RM - 1] = P::load(pw, o);
+                const unsigned o = lev + (unsigned)jn_m * nx;
+                wsu[2 * RM - 1] = (TC)ld(pu, o);
+                wsv[2 * RM - 1] = (TC)ld(pv, o);
+                wsw[2 * RM - 1] = (TC)ld(pw, o);
             }
         }
     }
@@ -387,20 +415,23 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
     using SFlux = typename P::SFlux;
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
+    constexpr int NC = 2 * (RM - 1);
     constexpr int H = (MOM && RM > RS) ? RM : RS;
     constexpr int NV = NSC + (MOM ? 3 : 1);  // u row always staged (advecting velocity)
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    TC *rows = reinterpret_cast<TC *>(smem_raw);
     const int nx = g.nx;
     const int W = nx + 2 * H;
-    SFlux *sflux = reinterpret_cast<SFlux *>(rows + NV * W);
-    TC *mflux = reinterpret_cast<TC *>(sflux + nx);
+    TC *rows = reinterpret_cast<TC *>(smem_raw);
+    const size_t fo = ((sizeof(TC) * (size_t)NV * W) + 15) & ~(size_t)15;
+    SFlux *sflux = reinterpret_cast<SFlux *>(smem_raw + fo);
+    const size_t mo_ = ((fo + sizeof(SFlux) * (size_t)nx) + 15) & ~(size_t)15;
+    TC *mflux = reinterpret_cast<TC *>(smem_raw + mo_);
     const int j = blockIdx.x;
     const int k = blockIdx.y + g.gw;
-    const long long row = ((long long)k * g.ny + j) * nx;
+    const unsigned plane = (unsigned)g.plane;
+    const unsigned row = ((unsigned)k * g.ny + j) * nx;
     const int tid = threadIdx.x;
     const int B = blockDim.x;
-    // slots: 0..NSC-1 scalars, NSC = u, NSC+1 = v, NSC+2 = w
     const TS *src[5];
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc) src[sc] = (const TS *)fl.q[sc];
@@ -410,7 +441,7 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
         src[NSC + 2] = (const TS *)fl.q[VW];
     }
     for (int v = 0; v < NV; ++v)
-        for (int t = tid; t < W; t += B) rows[v * W + t] = P::load(src[v], row + wrapi(t - H, nx));
+        for (int t = tid; t < W; t += B) rows[v * W + t] = (TC)ld(src[v], row + wrapi(t - H, nx));
     __syncthreads();
     const double rh = pr.rho0_half[k];
     const double r0 = pr.rho0[k];
@@ -419,29 +450,32 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
     const double dxi = g.dxi[0];
     const TC *urow = rows + NSC * W + H;  // urow[i] = u at cell i (wrapped halo)
 
-    // scalars
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc) {
-        const double b = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
-        double bw[2 * RS];
-#pragma unroll
-        for (int m = 0; m < 2 * RS; ++m) bw[m] = b;
+        double bw[1] = {P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0};
         const TC *srow = rows + sc * W + H;
         for (int c = tid; c < nx; c += B) {
             TC win[2 * RS];
 #pragma unroll
             for (int m = 0; m < 2 * RS; ++m) win[m] = srow[c - (RS - 1) + m];
-            sflux[c] = P::template scal_flux<SO>(win, bw, urow[c], rh);
+            sflux[c] = P::template scal_flux<SO, false>(win, bw, urow[c], rh);
         }
         __syncthreads();
         for (int c = tid; c < nx; c += B) {
             const int cm = c == 0 ? nx - 1 : c - 1;
-            P::acc((TS *)fl.t[sc], row + c, P::scal_div(sflux[c], sflux[cm], ah, dxi));
+            sub_into((TS *)fl.t[sc], row + c, P::scal_div(sflux[c], sflux[cm], ah, dxi), fl.tzero != 0);
         }
         __syncthreads();
     }
     if (MOM) {
         const TS *pu = (const TS *)fl.q[VU];
+        // row offsets of u at the centred y and z stencils (v and w advected by u)
+        int yo[NC], zo[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            yo[m] = (wrapi(j + m - (RM - 2), g.ny) - j) * nx;
+            zo[m] = (m - (RM - 2)) * (int)plane;
+        }
         for (int a = 0; a < 3; ++a) {  // advected component u, v, w
             const TC *prow = rows + (NSC + a) * W + H;
             const double rho = (a == 2) ? r0 : rh;
@@ -451,14 +485,15 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
                 for (int m = 0; m < 2 * RM; ++m) win[m] = prow[c - (RM - 1) + m];
                 TC U;
                 if (a == 0) {
-                    TC q[2 * (RM - 1)];
+                    TC q[NC];
 #pragma unroll
-                    for (int m = 0; m < 2 * (RM - 1); ++m) q[m] = urow[c + m - (RM - 2)];
+                    for (int m = 0; m < NC; ++m) q[m] = urow[c + m - (RM - 2)];
                     U = Interp<MO, TC>::centred(q);
-                } else if (a == 1) {
-                    U = centred_y<P, MO>(pu, g, c, j, k);
                 } else {
-                    U = centred_z<P, MO>(pu, g, c, j, k);
+                    unsigned o[NC];
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) o[m] = row + c + (a == 1 ? yo[m] : zo[m]);
+                    U = centred_at<P, MO>(pu, o);
                 }
                 mflux[c] = P::template mom_flux<MO>(win, U, rho);
             }
@@ -466,7 +501,8 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
             const double al = (a == 2) ? a0 : ah;
             for (int c = tid; c < nx; c += B) {
                 const int cm = c == 0 ? nx - 1 : c - 1;
-                P::acc((TS *)fl.t[VU + a], row + c, P::mom_div(mflux[c], mflux[cm], al, dxi));
+                sub_into((TS *)fl.t[VU + a], row + c, P::mom_div(mflux[c], mflux[cm], al, dxi),
+                         fl.tzero != 0 && a != 2);
             }
             __syncthreads();
         }
@@ -480,7 +516,9 @@ size_t x_smem_bytes(int nx) {
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int H = (MOM && RM > RS) ? RM : RS;
     constexpr int NV = NSC + (MOM ? 3 : 1);
-    return sizeof(TC) * (size_t)NV * (nx + 2 * H) + sizeof(typename P::SFlux) * nx + sizeof(TC) * nx + 64;
+    size_t fo = ((sizeof(TC) * (size_t)NV * (nx + 2 * H)) + 15) & ~(size_t)15;
+    size_t mo = ((fo + sizeof(typename P::SFlux) * (size_t)nx) + 15) & ~(size_t)15;
+    return mo + sizeof(TC) * nx + 16;
 }
 
 // Launch the three passes in the reference's per-variable order (x, y, z).

@@ -9,6 +9,8 @@
 #include <string.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/pycles_amd.h"
 
 // Physical constants, restated from reference generate_parameters.py:20-39.

@@ -42,7 +42,7 @@ struct pycles_pressure_plan {
     void *twy;
     double *kx2, *ky2, *rho_h, *dd, *a, *c;
     double *mean, *partial, *cfl;
-    int logx, logy;
+    int logx, logy, loggx;
 };
 
 static int ilog2(long long n) {
@@ -60,16 +60,18 @@ __device__ __forceinline__ unsigned brev(unsigned v, int bits) { return __brev(v
 // already in bit-reversed order.  tw[m] = exp(-2 pi i m/N), m < N/2; sign
 // +1 selects the inverse kernel (conjugate twiddles).  Butterflies are
 // evaluated as in a textbook DIT sweep: x_b <- x_a - w x_b, x_a <- x_a + w x_b.
+// batch and N are powers of two (logB = log2(batch)); tw points to LDS.
 template <typename T>
-__device__ void lds_fft(T *re, T *im, int N, int logN, int stride, int batch, const C2<T> *tw, int inverse) {
-    const int nb = (N >> 1) * batch;
+__device__ void lds_fft(T *re, T *im, int N, int logN, int stride, int logB, const C2<T> *tw, int inverse) {
+    const int batch = 1 << logB;
+    const int nb = (N >> 1) << logB;
     for (int s = 1; s <= logN; ++s) {
         const int h = 1 << (s - 1);
         const int tstep = N >> s;  // twiddle index stride
         for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-            const int b = t % batch;
-            const int bf = t / batch;
-            const int grp = bf / h, pos = bf % h;
+            const int b = t & (batch - 1);
+            const int bf = t >> logB;
+            const int grp = bf >> (s - 1), pos = bf & (h - 1);
             const int ia = (grp * 2 * h + pos) * stride + b;
             const int ib = ia + h * stride;
             const C2<T> w = tw[pos * tstep];
@@ -87,26 +89,32 @@ __device__ void lds_fft(T *re, T *im, int N, int logN, int stride, int batch, co
 }
 
 // ---------------------------------------------------------------------------
-// Level mean of w is computed before this kernel.  One block per (j, k') row:
-// divergence (PressureSolver.pyx:138-175, zeroed then x, y, z contributions
-// added in that order), forward FFT along x, half spectrum written out.
+// Level mean of w is computed before this kernel.  One block per R = 2^logR
+// consecutive rows (j0 .. j0+R-1, k'): divergence (PressureSolver.pyx:138-175,
+// zeroed then x, y, z contributions added in that order), forward FFT along x
+// of the R rows together (interleaved in LDS), half spectra written out.
 template <typename TS>
 __global__ __launch_bounds__(256) void div_fftx_kernel(Geo g, const TS *u, const TS *v, const TS *w,
                                                        const double *alpha0, const double *alpha0_half,
                                                        const double *mean, TS *div_out, C2<TS> *spec,
-                                                       const C2<TS> *tw, int logx, int nxh) {
+                                                       const C2<TS> *tw, int logx, int nxh, int logR) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    TS *re = reinterpret_cast<TS *>(smem_raw);
-    TS *im = re + g.nx;
-    const int j = blockIdx.x;
+    const int R = 1 << logR;
+    C2<TS> *twl = reinterpret_cast<C2<TS> *>(smem_raw);
+    TS *re = reinterpret_cast<TS *>(twl + g.nx / 2);
+    TS *im = re + g.nx * R;
+    for (int m = threadIdx.x; m < g.nx / 2; m += blockDim.x) twl[m] = tw[m];
     const int kk = blockIdx.y;
     const int k = kk + g.gw;
-    const int jm = j == 0 ? g.ny - 1 : j - 1;
     const double ah = alpha0_half[k];
     const double a0 = alpha0[k], a0m = alpha0[k - 1];
     const double mk = mean[k], mkm = mean[k - 1];
     const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1], dxi2 = 1.0 / g.dx[2];
-    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+    for (int t = threadIdx.x; t < (g.nx << logR); t += blockDim.x) {
+        const int i = t & (g.nx - 1);
+        const int rb = t >> logx;
+        const int j = (blockIdx.x << logR) + rb;
+        const int jm = j == 0 ? g.ny - 1 : j - 1;
         const int im1 = i == 0 ? g.nx - 1 : i - 1;
         const long long o = gidx(g, i, j, k);
         TS d;
@@ -129,29 +137,34 @@ __global__ __launch_bounds__(256) void div_fftx_kernel(Geo g, const TS *u, const
             d = (TS)dv;
         }
         if (div_out) div_out[o] = d;
-        const unsigned r = brev((unsigned)i, logx);
+        const unsigned r = (brev((unsigned)i, logx) << logR) + rb;
         re[r] = d;
         im[r] = TS(0);
     }
     __syncthreads();
-    lds_fft<TS>(re, im, g.nx, logx, 1, 1, tw, 0);
-    C2<TS> *row = spec + ((long long)kk * g.ny + j) * nxh;
-    for (int m = threadIdx.x; m < nxh; m += blockDim.x) row[m] = C2<TS>{re[m], im[m]};
+    lds_fft<TS>(re, im, g.nx, logx, R, logR, twl, 0);
+    for (int rb = 0; rb < R; ++rb) {
+        const int j = (blockIdx.x << logR) + rb;
+        C2<TS> *row = spec + ((long long)kk * g.ny + j) * nxh;
+        for (int m = threadIdx.x; m < nxh; m += blockDim.x) row[m] = C2<TS>{re[(m << logR) + rb], im[(m << logR) + rb]};
+    }
 }
 
 // FFT along y for gx consecutive kx columns of one level; in place.
 template <typename TS>
 __global__ __launch_bounds__(256) void ffty_kernel(Geo g, C2<TS> *spec, const C2<TS> *tw, int logy, int nxh,
-                                                   int gx, int inverse) {
+                                                   int gx, int loggx, int inverse) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    TS *re = reinterpret_cast<TS *>(smem_raw);
+    C2<TS> *twl = reinterpret_cast<C2<TS> *>(smem_raw);
+    TS *re = reinterpret_cast<TS *>(twl + g.ny / 2);
     TS *im = re + (long long)g.ny * gx;
+    for (int m = threadIdx.x; m < g.ny / 2; m += blockDim.x) twl[m] = tw[m];
     const int kx0 = blockIdx.x * gx;
     const int kk = blockIdx.y;
     const int ncol = min(gx, nxh - kx0);
     C2<TS> *lev = spec + (long long)kk * g.ny * nxh;
     for (int t = threadIdx.x; t < g.ny * gx; t += blockDim.x) {
-        const int y = t / gx, b = t % gx;
+        const int y = t >> loggx, b = t & (gx - 1);
         const unsigned r = brev((unsigned)y, logy);
         if (b < ncol) {
             const C2<TS> z = lev[(long long)y * nxh + kx0 + b];
@@ -163,10 +176,10 @@ __global__ __launch_bounds__(256) void ffty_kernel(Geo g, C2<TS> *spec, const C2
         }
     }
     __syncthreads();
-    lds_fft<TS>(re, im, g.ny, logy, gx, gx, tw, inverse);
+    lds_fft<TS>(re, im, g.ny, logy, gx, loggx, twl, inverse);
     const TS sc = inverse ? TS(1.0 / (double)g.ny) : TS(1);
     for (int t = threadIdx.x; t < g.ny * gx; t += blockDim.x) {
-        const int y = t / gx, b = t % gx;
+        const int y = t >> loggx, b = t & (gx - 1);
         if (b < ncol) {
             TS xr = re[y * gx + b], xi = im[y * gx + b];
             if (inverse) {
@@ -232,30 +245,38 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double
 // its mirrored z ghost level (sym, as communicate_variable does).
 template <typename TS>
 __global__ __launch_bounds__(256) void ifftx_kernel(Geo g, const C2<TS> *spec, TS *p, const C2<TS> *tw, int logx,
-                                                    int nxh) {
+                                                    int nxh, int logR) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    TS *re = reinterpret_cast<TS *>(smem_raw);
-    TS *im = re + g.nx;
-    const int j = blockIdx.x;
+    const int R = 1 << logR;
+    C2<TS> *twl = reinterpret_cast<C2<TS> *>(smem_raw);
+    TS *re = reinterpret_cast<TS *>(twl + g.nx / 2);
+    TS *im = re + g.nx * R;
+    for (int m = threadIdx.x; m < g.nx / 2; m += blockDim.x) twl[m] = tw[m];
     const int kk = blockIdx.y;
-    const C2<TS> *row = spec + ((long long)kk * g.ny + j) * nxh;
-    for (int m = threadIdx.x; m < g.nx; m += blockDim.x) {
+    for (int t = threadIdx.x; t < (g.nx << logR); t += blockDim.x) {
+        const int m = t & (g.nx - 1);
+        const int rb = t >> logx;
+        const int j = (blockIdx.x << logR) + rb;
+        const C2<TS> *row = spec + ((long long)kk * g.ny + j) * nxh;
         C2<TS> z;
         if (m < nxh) z = row[m];
         else {
             z = row[g.nx - m];
             z.y = -z.y;
         }
-        const unsigned r = brev((unsigned)m, logx);
+        const unsigned r = (brev((unsigned)m, logx) << logR) + rb;
         re[r] = z.x;
         im[r] = z.y;
     }
     __syncthreads();
-    lds_fft<TS>(re, im, g.nx, logx, 1, 1, tw, 1);
+    lds_fft<TS>(re, im, g.nx, logx, R, logR, twl, 1);
     const TS sc = TS(1.0 / (double)g.nx);
     const int k = kk + g.gw;
-    for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
-        const TS val = re[i] * sc;
+    for (int t = threadIdx.x; t < (g.nx << logR); t += blockDim.x) {
+        const int i = t & (g.nx - 1);
+        const int rb = t >> logx;
+        const int j = (blockIdx.x << logR) + rb;
+        const TS val = re[(i << logR) + rb] * sc;
         p[gidx(g, i, j, k)] = val;
         if (kk < g.gw) p[gidx(g, i, j, g.gw - 1 - kk)] = val;
         if (kk >= g.nz - g.gw) p[gidx(g, i, j, 2 * g.gw + 2 * g.nz - 1 - k)] = val;
@@ -380,8 +401,9 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     const size_t cs = precision == PYCLES_F64 ? 16 : 8;
     const size_t es = cs / 2;
     int gx = 16;
-    while (gx > 1 && (size_t)2 * g.ny * gx * es > 64 * 1024) gx >>= 1;
+    while (gx > 1 && (size_t)2 * g.ny * gx * es + (size_t)g.ny / 2 * cs > 64 * 1024) gx >>= 1;
     P->gx = gx;
+    P->loggx = ilog2(gx);
     const long long nspec = (long long)g.nz * g.ny * P->nxh;
     const int gw = g.gw, nz = g.nz;
     // host tables (same expressions as the reference initializers)
@@ -480,26 +502,30 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
     const Geo &g = P->g;
     int rc = pycles_level_mean_ws(g, w, P->mean, P->partial, nullptr, P->precision, st);
     if (rc) return rc;
-    const size_t rowsmem = 2 * sizeof(TS) * g.nx;
-    const int bx = g.nx >= 256 ? 256 : ((g.nx + 63) / 64) * 64;
-    hipLaunchKernelGGL(div_fftx_kernel<TS>, dim3(g.ny, g.nz), dim3(bx), rowsmem, st, g, (const TS *)u,
+    // rows per block for the x transforms: about 2048 points per block
+    int logR = 0;
+    while ((g.nx << (logR + 1)) <= 2048 && (1 << (logR + 1)) <= g.ny && logR < 4) ++logR;
+    const int R = 1 << logR;
+    const size_t rowsmem = 2 * sizeof(TS) * g.nx * R + sizeof(C2<TS>) * (g.nx / 2);
+    const int bx = 256;
+    hipLaunchKernelGGL(div_fftx_kernel<TS>, dim3(g.ny / R, g.nz), dim3(bx), rowsmem, st, g, (const TS *)u,
                        (const TS *)v, (const TS *)w, ref->alpha0, ref->alpha0_half, P->mean, (TS *)div,
-                       (C2<TS> *)P->spec, (const C2<TS> *)P->twx, P->logx, P->nxh);
+                       (C2<TS> *)P->spec, (const C2<TS> *)P->twx, P->logx, P->nxh, logR);
     PYCLES_CHECK_LAUNCH();
-    const size_t ysmem = 2 * sizeof(TS) * (size_t)g.ny * P->gx;
+    const size_t ysmem = 2 * sizeof(TS) * (size_t)g.ny * P->gx + sizeof(C2<TS>) * (g.ny / 2);
     const dim3 ygrid((P->nxh + P->gx - 1) / P->gx, g.nz);
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(256), ysmem, st, g, (C2<TS> *)P->spec,
-                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, 0);
+                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 0);
     PYCLES_CHECK_LAUNCH();
     const long long ncol = (long long)g.ny * P->nxh;
     hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
                        (C2<TS> *)P->spec, P->xs, P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c, P->nxh);
     PYCLES_CHECK_LAUNCH();
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(256), ysmem, st, g, (C2<TS> *)P->spec,
-                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, 1);
+                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 1);
     PYCLES_CHECK_LAUNCH();
-    hipLaunchKernelGGL(ifftx_kernel<TS>, dim3(g.ny, g.nz), dim3(bx), rowsmem, st, g, (const C2<TS> *)P->spec,
-                       (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh);
+    hipLaunchKernelGGL(ifftx_kernel<TS>, dim3(g.ny / R, g.nz), dim3(bx), rowsmem, st, g, (const C2<TS> *)P->spec,
+                       (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, logR);
     PYCLES_CHECK_LAUNCH();
     if (cfl_out) hipLaunchKernelGGL(cfl_zero_kernel, dim3(1), dim3(1), 0, st, P->cfl);
     hipLaunchKernelGGL(correct_kernel<TS>, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, st, g, (TS *)u, (TS *)v,

@@ -1,8 +1,12 @@
 // Thermodynamics kernels: saturation adjustment (SA) and dry.
 //
-// One thread per (i, j) column marching k over all nzg levels, so that the
-// vertical neighbours needed by the buoyancy average into the w tendency and
-// by the buoyancy frequency are this thread's own previous levels.
+// Each thread marches one (i, j) column in k, so the vertical neighbours
+// needed by the buoyancy average into the w tendency and by the buoyancy
+// frequency are its own previous levels.  Columns are split into z chunks
+// (grid z) for parallelism: a thread owns levels [k0, k1), starts one level
+// early to fill its windows and reads one level past k1; it writes only the
+// outputs of the levels it owns.  The exner function of every level is
+// computed once per block into LDS.
 //
 //   ThermodynamicsSA.update       ThermodynamicsSA.pyx:176-213
 //   clip_qt                       thermodynamics_sa.h:213-220
@@ -14,9 +18,12 @@
 //   lookup                        lookup.h:48-54 (table staged in LDS)
 //   latent heat                   Microphysics.pxd:65-75
 #include "common.h"
+#include "mlp_infer.h"
+#include "thermo_f32.h"
 #include "thermo_funcs.h"
 
 #define LOOKUP_MAX 2048
+#define LEVELS_MAX 2048
 
 struct SAArgs {
     const double *p0, *alpha0_half;
@@ -25,109 +32,262 @@ struct SAArgs {
     double x_min, dx, dxi;
     void *s, *qt, *wt, *T, *qv, *ql, *alpha, *buoy, *thr, *bvf, *thl;
     const double *sbase, *qbase;
-    int clip, max_iter, lh;
-    const void *mlp;
+    int clip, max_iter, lh, wt_over;
+    const mlp::Weights *mlp;
     int newton;
     unsigned long long *counters;
+    int kchunk;
 };
 
-template <typename TS>
-__global__ __launch_bounds__(256) void thermo_sa_kernel(Geo g, SAArgs a) {
+// Block-level reduction of the four counters into out.
+__device__ __forceinline__ void flush_counters(unsigned long long *cnt, const unsigned long long v[4],
+                                               unsigned long long *out, int tid) {
+    if (!out) return;
+    for (int c = 0; c < 4; ++c)
+        if (v[c]) atomicAdd(&cnt[c], v[c]);
+    __syncthreads();
+    if (tid < 4 && cnt[tid]) atomicAdd(&out[tid], cnt[tid]);
+}
+
+// z chunk of this block: owned [k0, k1), marched [kw, ke)
+struct Chunk {
+    int k0, k1, kw, ke;
+};
+__device__ __forceinline__ Chunk chunk_of(const Geo &g, int kchunk) {
+    Chunk c;
+    c.k0 = blockIdx.z * kchunk;
+    c.k1 = min(c.k0 + kchunk, g.nzg);
+    c.kw = max(0, c.k0 - 1);
+    c.ke = min(c.k1 + 1, g.nzg);
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// FP64: the reference's arithmetic.
+__global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
     __shared__ double tab[LOOKUP_MAX];
+    __shared__ double exner[LEVELS_MAX];
     __shared__ unsigned long long cnt[4];
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    for (int t = tid; t < a.ntab; t += blockDim.x * blockDim.y) tab[t] = a.tab[t];
+    const int nthr = blockDim.x * blockDim.y;
+    const Chunk ch = chunk_of(g, a.kchunk);
+    for (int t = tid; t < a.ntab; t += nthr) tab[t] = a.tab[t];
+    for (int k = ch.kw + tid; k < ch.ke; k += nthr) exner[k] = thermo::exner_c(a.p0[k]);
     if (tid < 4) cnt[tid] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    unsigned long long n_sat = 0, n_it = 0, n_cap = 0;
+    unsigned long long ctr[4] = {0, 0, 0, 0};
     if (i < g.nx && j < g.ny) {
         const long long col = (long long)j * g.nx + i;
-        TS *s = (TS *)a.s;
-        TS *qtp = (TS *)a.qt;
-        TS *wt = (TS *)a.wt;
-        TS *To = (TS *)a.T;
-        TS *qvo = (TS *)a.qv;
-        TS *qlo = (TS *)a.ql;
-        TS *alo = (TS *)a.alpha;
-        TS *bo = (TS *)a.buoy;
-        TS *thro = (TS *)a.thr;
-        TS *bvfo = (TS *)a.bvf;
-        TS *thlo = (TS *)a.thl;
+        double *s = (double *)a.s;
+        double *qtp = (double *)a.qt;
+        double *wt = (double *)a.wt;
+        double *To = (double *)a.T, *qvo = (double *)a.qv, *qlo = (double *)a.ql, *alo = (double *)a.alpha;
+        double *bo = (double *)a.buoy, *thro = (double *)a.thr, *bvfo = (double *)a.bvf, *thlo = (double *)a.thl;
         const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
         const double dzi = 1.0 / g.dx[2];
-        // windows of the two previous levels (index 0 = k-2, 1 = k-1)
-        double Tw[3], qtw[3], qvw[3], thw[3], bw[2];
-        for (int k = 0; k < g.nzg; ++k) {
+        double Tw[3] = {0, 0, 0}, qtw[3] = {0, 0, 0}, qvw[3] = {0, 0, 0}, thw[3] = {0, 0, 0}, bprev = 0.0;
+        for (int k = ch.kw; k < ch.ke; ++k) {
+            const bool own = k >= ch.k0 && k < ch.k1;
+            const bool own_m = k - 1 >= ch.k0 && k - 1 < ch.k1;
             const long long o = (long long)k * g.plane + col;
             const double p0 = a.p0[k];
-            double qt = (double)qtp[o];
-            if (a.qbase) qt += a.qbase[k];
+            double qt = qtp[o];
             if (a.clip) {
                 const double c = fmax(qt, 1e-11);
                 if (!(c == qt)) {
                     qt = c;
-                    qtp[o] = (TS)(a.qbase ? c - a.qbase[k] : c);
+                    if (own) qtp[o] = c;
                 }
             }
-            double sv = (double)s[o];
-            if (a.sbase) sv += a.sbase[k];
             double T, qv, ql, qi;
-            const int it = thermo::eos_sa(lt, p0, sv, qt, &T, &qv, &ql, &qi, a.max_iter);
-            if (it > 0) {
-                n_sat += 1;
-                n_it += it;
-                if (a.max_iter > 0 && it >= a.max_iter) n_cap += 1;
+            const int it = thermo::eos_sa(lt, p0, s[o], qt, &T, &qv, &ql, &qi, a.max_iter);
+            if (it > 0 && own) {
+                ctr[0] += 1;
+                ctr[1] += it;
+                if (a.max_iter > 0 && it >= a.max_iter) ctr[2] += 1;
             }
             const double al = thermo::alpha_c(p0, T, qt, qv);
-            To[o] = (TS)T;
-            qvo[o] = (TS)qv;
-            qlo[o] = (TS)ql;
-            alo[o] = (TS)al;
-            // buoyancy for k in [0, nzg-1); w tendency at k-1 for k-1 in [1, nzg-3)
             double b = 0.0;
-            if (k < g.nzg - 1) {
-                b = thermo::buoyancy_c(a.alpha0_half[k], al);
-                if (bo) bo[o] = (TS)b;
-            }
-            if (k >= 2 && k - 1 < g.nzg - 3) {
-                const long long om = o - g.plane;
-                wt[om] = (TS)((double)wt[om] + 0.5 * (bw[1] + b));
-            }
-            // theta_rho for k in [1, nzg-1)
+            if (k < g.nzg - 1) b = thermo::buoyancy_c(a.alpha0_half[k], al);
+            const double ex = exner[k];
             double thr = 0.0;
-            if (k >= 1 && k < g.nzg - 1) {
-                thr = thermo::theta_rho_c(p0, T, qt, qv);
-                if (thro) thro[o] = (TS)thr;
+            if (k >= 1 && k < g.nzg - 1) thr = T * (1.0 - qt + PC_EPS_VI * qv) / ex;
+            if (own) {
+                To[o] = T;
+                qvo[o] = qv;
+                qlo[o] = ql;
+                alo[o] = al;
+                if (bo && k < g.nzg - 1) bo[o] = b;
+                if (thro && k >= 1 && k < g.nzg - 1) thro[o] = thr;
+                if (thlo) {
+                    // exp(-L*0/(T cpd)) is exactly 1 for unsaturated cells
+                    if (ql == 0.0 && qi == 0.0) thlo[o] = T / ex * 1.0;
+                    else {
+                        const double L = thermo::latent_heat_variable(T);
+                        thlo[o] = T / ex * exp(-L * (ql / (1.0 - qt) + qi / (1.0 - qt)) / (T * PC_CPD));
+                    }
+                }
             }
-            // shift windows: slot 2 = level k
+            // w tendency at k-1 for k-1 in [1, nzg-3): wt + interp_2(b[k-1], b[k])
+            if (own_m && k >= 2 && k - 1 < g.nzg - 3) {
+                const double w0 = a.wt_over ? 0.0 : wt[o - g.plane];
+                wt[o - g.plane] = w0 + 0.5 * (bprev + b);
+            }
+            bprev = b;
             Tw[0] = Tw[1]; Tw[1] = Tw[2]; Tw[2] = T;
             qtw[0] = qtw[1]; qtw[1] = qtw[2]; qtw[2] = qt;
             qvw[0] = qvw[1]; qvw[1] = qvw[2]; qvw[2] = qv;
             thw[0] = thw[1]; thw[1] = thw[2]; thw[2] = thr;
-            bw[0] = bw[1]; bw[1] = b;
-            // bvf at m = k-1 for m in [2, nzg-2)
             const int m = k - 1;
-            if (bvfo && m >= 2 && m < g.nzg - 2) {
-                bvfo[o - g.plane] = (TS)thermo::bvf_sa(lt, a.p0[m], Tw, qtw, qvw, thw, dzi);
-            }
-            if (thlo) {
-                const double L = thermo::latent_heat_variable(T);
-                thlo[o] = (TS)thermo::thetali_c(p0, T, qt, ql, qi, L);
-            }
+            if (bvfo && own_m && m >= 2 && m < g.nzg - 2)
+                bvfo[o - g.plane] = thermo::bvf_sa(lt, a.p0[m], Tw, qtw, qvw, thw, dzi);
         }
     }
-    if (a.counters) {
-        atomicAdd(&cnt[0], n_sat);
-        atomicAdd(&cnt[1], n_it);
-        atomicAdd(&cnt[2], n_cap);
-        __syncthreads();
-        if (tid < 3 && cnt[tid]) atomicAdd(&a.counters[tid], cnt[tid]);
+    flush_counters(cnt, ctr, a.counters, tid);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 storage and arithmetic; s and qt are deviations from FP64 level
+// profiles; FP64 only in the per-level constants and the residual check.
+// MLP selects the learned first guess (a separate instantiation, so the plain
+// kernel carries none of the network's registers).
+template <bool MLP>
+__global__ __launch_bounds__(256) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
+    __shared__ float tabf[LOOKUP_MAX];
+    __shared__ double tab[LOOKUP_MAX];
+    __shared__ float exner[LEVELS_MAX];
+    __shared__ unsigned long long cnt[4];
+    struct NoSmem {};
+    __shared__ typename std::conditional<MLP, mlp::Smem, NoSmem>::type msm_store;
+    mlp::Smem *msm = reinterpret_cast<mlp::Smem *>(&msm_store);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nthr = blockDim.x * blockDim.y;
+    const Chunk ch = chunk_of(g, a.kchunk);
+    for (int t = tid; t < a.ntab; t += nthr) {
+        tab[t] = a.tab[t];
+        tabf[t] = (float)a.tab[t];
     }
+    for (int k = ch.kw + tid; k < ch.ke; k += nthr) exner[k] = (float)thermo::exner_c(a.p0[k]);
+    if (tid < 4) cnt[tid] = 0;
+    if (MLP) mlp::stage(a.mlp, msm, tid, nthr);
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    const bool active = i < g.nx && j < g.ny;
+    const long long col = active ? (long long)j * g.nx + i : 0;
+    float *s = (float *)a.s;
+    float *qtp = (float *)a.qt;
+    float *wt = (float *)a.wt;
+    float *To = (float *)a.T, *qvo = (float *)a.qv, *qlo = (float *)a.ql, *alo = (float *)a.alpha;
+    float *bo = (float *)a.buoy, *thro = (float *)a.thr, *bvfo = (float *)a.bvf, *thlo = (float *)a.thl;
+    const thermo32::TableF ltf{tabf, (float)a.x_min, (float)a.dx, (float)a.dxi, a.ntab};
+    const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
+    const float dzi = (float)(1.0 / g.dx[2]);
+    float Tw[3] = {0, 0, 0}, qtw[3] = {0, 0, 0}, qvw[3] = {0, 0, 0}, thw[3] = {0, 0, 0}, bprev = 0.0f;
+    for (int k = ch.kw; k < ch.ke; ++k) {
+        const bool own = active && k >= ch.k0 && k < ch.k1;
+        const bool own_m = active && k - 1 >= ch.k0 && k - 1 < ch.k1;
+        const long long o = (long long)k * g.plane + col;
+        const double p0 = a.p0[k];
+        const float p0f = (float)p0;
+        const double sb = a.sbase ? a.sbase[k] : 0.0;
+        const double qb = a.qbase ? a.qbase[k] : 0.0;
+        float sd = active ? s[o] : 0.0f;
+        float qd = active ? qtp[o] : 0.0f;
+        double qt64 = qb + (double)qd;
+        if (a.clip && !(qt64 >= 1e-11)) {
+            qt64 = 1e-11;
+            if (own) qtp[o] = (float)(1e-11 - qb);
+        }
+        const float qt = (float)qt64;
+        const double s64 = sb + (double)sd;
+        // s - (1-qt) sd~ - qt sv~ formed from the level profile and the deviations
+        const double s_red = (sb - PC_SD_TILDE - qb * (PC_SV_TILDE - PC_SD_TILDE)) + (double)sd -
+                             (qt64 - qb) * (PC_SV_TILDE - PC_SD_TILDE);
+        float T_guess = -1.0f;
+        if (MLP) T_guess = mlp::guess(msm, (float)s64, qt, p0f);
+        float T, qv, ql;
+        int fb = 0;
+        const int it = thermo32::eos_sa_f32(ltf, lt, p0f, p0, s64, s_red, qt, qt64, T_guess, &T, &qv, &ql,
+                                            a.max_iter, &fb);
+        if (it > 0 && own) {
+            ctr[0] += 1;
+            ctr[1] += it;
+            if (a.max_iter > 0 && it >= a.max_iter) ctr[2] += 1;
+            ctr[3] += fb;
+        }
+        const float al = (float)PC_RD * T / p0f * (1.0f - qt + (float)PC_EPS_VI * qv);
+        const float ex = exner[k];
+        float b = 0.0f;
+        if (k < g.nzg - 1) {
+            const float a0 = (float)a.alpha0_half[k];
+            b = (float)PC_G * (al - a0) / a0;
+        }
+        float thr = 0.0f;
+        if (k >= 1 && k < g.nzg - 1) thr = T * (1.0f - qt + (float)PC_EPS_VI * qv) / ex;
+        Tw[0] = Tw[1]; Tw[1] = Tw[2]; Tw[2] = T;
+        qtw[0] = qtw[1]; qtw[1] = qtw[2]; qtw[2] = qt;
+        qvw[0] = qvw[1]; qvw[1] = qvw[2]; qvw[2] = qv;
+        thw[0] = thw[1]; thw[1] = thw[2]; thw[2] = thr;
+        if (own) {
+            To[o] = T;
+            qvo[o] = qv;
+            qlo[o] = ql;
+            alo[o] = al;
+            if (bo && k < g.nzg - 1) bo[o] = b;
+            if (thro && k >= 1 && k < g.nzg - 1) thro[o] = thr;
+            if (thlo) {
+                if (ql == 0.0f) thlo[o] = T / ex;
+                else {
+                    const float L = thermo32::latent_heat(T);
+                    thlo[o] = T / ex * expf(-L * (ql / (1.0f - qt)) / (T * (float)PC_CPD));
+                }
+            }
+        }
+        if (own_m && k >= 2 && k - 1 < g.nzg - 3) {
+            const float w0 = a.wt_over ? 0.0f : wt[o - g.plane];
+            wt[o - g.plane] = w0 + 0.5f * (bprev + b);
+        }
+        bprev = b;
+        const int m = k - 1;
+        if (bvfo && own_m && m >= 2 && m < g.nzg - 2) {
+            float bv;
+            if (qvw[1] < qtw[1]) {
+                const float Tk = Tw[1];
+                const float Lv = thermo32::latent_heat(Tk);
+                const float pvs = thermo32::lookup(ltf, Tk);
+                const float p0m = (float)a.p0[m];
+                const float rsl = (float)PC_EPS_V * pvs / (p0m - pvs);
+                const float cpm = (1.0f - qtw[1]) * (float)PC_CPD + qtw[1] * (float)PC_CPV;
+                const float gamma_w = (float)(PC_G / PC_CPD) * (1.0f / (1.0f - qtw[1])) *
+                                      (1.0f + Lv * rsl / ((float)PC_RD * Tk)) /
+                                      (cpm / (float)PC_CPD + Lv * Lv * ((float)PC_EPS_V + rsl) * rsl /
+                                                                 ((float)(PC_CPD * PC_RD) * Tk * Tk));
+                const float dTdz = 0.5f * (Tw[2] - Tw[0]) * dzi;
+                const float dqtdz = 0.5f * (qtw[2] - qtw[0]) * dzi;
+                bv = (float)PC_G / Tk * (dTdz + gamma_w) * (1.0f + Lv * rsl / ((float)PC_RD * Tk)) -
+                     dqtdz / (1.0f - qtw[1]);
+            } else {
+                bv = (float)PC_G / thw[1] * 0.5f * (thw[2] - thw[0]) * dzi;
+            }
+            bvfo[o - g.plane] = bv;
+        }
+    }
+    flush_counters(cnt, ctr, a.counters, tid);
 }
 
 extern int pycles_check_dims(const pycles_dims *d);
+
+// Chunks of the column so the grid holds at least ~4096 blocks.
+static int thermo_kchunk(const Geo &g) {
+    const long long cols = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
+    int nch = 1;
+    while (cols * nch < 4096 && g.nzg / (nch * 2) >= 16) nch *= 2;
+    return (g.nzg + nch - 1) / nch;
+}
 
 extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt, void *s,
                                 void *qt, void *wt, void *T, void *qv, void *ql, void *alpha, void *buoyancy,
@@ -148,6 +308,14 @@ extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, 
         return PYCLES_ERR_UNSUPPORTED;
     }
     Geo g = make_geo(dims);
+    if (g.nzg > LEVELS_MAX) {
+        pycles_set_error("thermo_sa: %d levels exceed %d", g.nzg, LEVELS_MAX);
+        return PYCLES_ERR_UNSUPPORTED;
+    }
+    if (opt->mlp && precision != PYCLES_F32) {
+        pycles_set_error("thermo_sa: the learned first guess is part of the FP32 form only");
+        return PYCLES_ERR_ARG;
+    }
     SAArgs a;
     a.p0 = ref->p0_half;
     a.alpha0_half = ref->alpha0_half;
@@ -163,15 +331,20 @@ extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, 
     a.clip = opt->do_qt_clipping;
     a.max_iter = opt->max_iter;
     a.lh = opt->latent_heat;
-    a.mlp = opt->mlp;
+    a.wt_over = opt->wt_overwrite;
+    a.mlp = (const mlp::Weights *)opt->mlp;
     a.newton = opt->newton_steps;
     a.counters = (unsigned long long *)opt->counters;
-    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4);
+    a.kchunk = thermo_kchunk(g);
+    const int nch = (g.nzg + a.kchunk - 1) / a.kchunk;
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, nch);
     hipStream_t st = (hipStream_t)stream;
     if (precision == PYCLES_F64)
-        hipLaunchKernelGGL(thermo_sa_kernel<double>, grd, blk, 0, st, g, a);
+        hipLaunchKernelGGL(thermo_sa_f64_kernel, grd, blk, 0, st, g, a);
+    else if (precision == PYCLES_F32 && a.mlp)
+        hipLaunchKernelGGL(thermo_sa_f32_kernel<true>, grd, blk, 0, st, g, a);
     else if (precision == PYCLES_F32)
-        hipLaunchKernelGGL(thermo_sa_kernel<float>, grd, blk, 0, st, g, a);
+        hipLaunchKernelGGL(thermo_sa_f32_kernel<false>, grd, blk, 0, st, g, a);
     else {
         pycles_set_error("bad precision %d", precision);
         return PYCLES_ERR_ARG;
@@ -185,6 +358,7 @@ struct DryArgs {
     const double *p0, *alpha0_half;
     void *s, *wt, *T, *alpha, *buoy, *theta, *bvf;
     const double *sbase;
+    int wt_over, kchunk;
 };
 
 template <typename TS>
@@ -192,6 +366,7 @@ __global__ __launch_bounds__(256) void thermo_dry_kernel(Geo g, DryArgs a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.nx || j >= g.ny) return;
+    const Chunk ch = chunk_of(g, a.kchunk);
     const long long col = (long long)j * g.nx + i;
     TS *s = (TS *)a.s;
     TS *wt = (TS *)a.wt;
@@ -202,41 +377,42 @@ __global__ __launch_bounds__(256) void thermo_dry_kernel(Geo g, DryArgs a) {
     TS *bvfo = (TS *)a.bvf;
     const double dzi = 1.0 / g.dx[2];
     double thw[3] = {0.0, 0.0, 0.0}, bprev = 0.0;
-    for (int k = 0; k < g.nzg; ++k) {
+    for (int k = ch.kw; k < ch.ke; ++k) {
+        const bool own = k >= ch.k0 && k < ch.k1;
+        const bool own_m = k - 1 >= ch.k0 && k - 1 < ch.k1;
         const long long o = (long long)k * g.plane + col;
         const double p0 = a.p0[k];
         double sv = (double)s[o];
         if (a.sbase) sv += a.sbase[k];
         const double T = thermo::eos_dry(p0, sv);
         const double al = thermo::alpha_c(p0, T, 0.0, 0.0);
-        To[o] = (TS)T;
-        alo[o] = (TS)al;
         double b = 0.0;
-        if (k < g.nzg - 1) {
-            b = thermo::buoyancy_c(a.alpha0_half[k], al);
-            if (bo) bo[o] = (TS)b;
+        if (k < g.nzg - 1) b = thermo::buoyancy_c(a.alpha0_half[k], al);
+        double th = 0.0;
+        if (k >= 1) th = T / thermo::exner_c(p0);
+        if (own) {
+            To[o] = (TS)T;
+            alo[o] = (TS)al;
+            if (bo && k < g.nzg - 1) bo[o] = (TS)b;
+            if (tho && k >= 1) tho[o] = (TS)th;
         }
         // wt[k-1] += 0.5*(b[k-1] + b[k]) for k-1 in [0, nzg-2)
-        if (k >= 1 && k - 1 < g.nzg - 2) {
+        if (own_m && k >= 1 && k - 1 < g.nzg - 2) {
             const long long om = o - g.plane;
-            wt[om] = (TS)((double)wt[om] + 0.5 * (bprev + b));
+            const double w0 = a.wt_over ? 0.0 : (double)wt[om];
+            wt[om] = (TS)(w0 + 0.5 * (bprev + b));
         }
         bprev = b;
-        double th = 0.0;
-        if (k >= 1) {
-            th = T / thermo::exner_c(p0);
-            if (tho) tho[o] = (TS)th;
-        }
         thw[0] = thw[1]; thw[1] = thw[2]; thw[2] = th;
         const int m = k - 1;
-        if (bvfo && m >= 2 && m < g.nzg - 1)
+        if (bvfo && own_m && m >= 2 && m < g.nzg - 1)
             bvfo[o - g.plane] = (TS)(PC_G / thw[1] * (0.5 * (thw[1] + thw[2]) - 0.5 * (thw[0] + thw[1])) * dzi);
     }
 }
 
 extern "C" int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref, void *s, void *wt, void *T,
                                  void *alpha, void *buoyancy, void *theta, void *bvf,
-                                 const pycles_scalar_base *base, int precision, void *stream) {
+                                 const pycles_scalar_base *base, int precision, int wt_overwrite, void *stream) {
     int rc = pycles_check_dims(dims);
     if (rc) return rc;
     if (!ref || !s || !wt || !T || !alpha) {
@@ -249,7 +425,10 @@ extern "C" int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref,
     a.alpha0_half = ref->alpha0_half;
     a.s = s; a.wt = wt; a.T = T; a.alpha = alpha; a.buoy = buoyancy; a.theta = theta; a.bvf = bvf;
     a.sbase = (precision == PYCLES_F32 && base) ? base->base[0] : nullptr;
-    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4);
+    a.wt_over = wt_overwrite;
+    a.kchunk = thermo_kchunk(g);
+    const int nch = (g.nzg + a.kchunk - 1) / a.kchunk;
+    dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, nch);
     hipStream_t st = (hipStream_t)stream;
     if (precision == PYCLES_F64)
         hipLaunchKernelGGL(thermo_dry_kernel<double>, grd, blk, 0, st, g, a);

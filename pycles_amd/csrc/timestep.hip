@@ -84,29 +84,51 @@ struct PtrPack {
 
 // One thread per column: interior update in k, then the ghost fill of the
 // same column, so no thread reads a level another thread writes.
-template <typename T>
-__global__ __launch_bounds__(256) void rk3_kernel(Geo g, int nv, PtrPack pk, int rk_step, double dt,
-                                                  int zero_tend) {
+// Grid (columns/256, nv): blockIdx.y selects the variable.
+template <typename T, int RK>
+__device__ __forceinline__ T rk_update(T phi, T tn, T cp, T dt) {
+    if (RK == 0) return phi + tn * dt;
+    if (RK == 1) return T(0.75) * cp + T(0.25) * (phi + tn * dt);
+    return T(1.0 / 3.0) * cp + T(2.0 / 3.0) * (phi + tn * dt);
+}
+
+template <typename T, int RK>
+__global__ __launch_bounds__(256) void rk3_kernel(Geo g, PtrPack pk, double dtd, int zero_tend) {
     const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= g.plane) return;
-    for (int v = 0; v < nv; ++v) {
+    const int v = blockIdx.y;
+    const T dt = (T)dtd;
+    {
         T *val = (T *)pk.v[v];
         T *ten = (T *)pk.t[v];
         T *cpy = (T *)pk.c[v];
-        for (int k = g.gw; k < g.nz + g.gw; ++k) {
-            const long long o = (long long)k * g.plane + col;
+        const long long P = g.plane;
+        int k = g.gw;
+        const int k1 = g.nz + g.gw;
+        // four levels per iteration: independent loads in flight
+        for (; k + 3 < k1; k += 4) {
+            const long long o = (long long)k * P + col;
+            T phi[4], tn[4], cp[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                phi[u] = val[o + u * P];
+                tn[u] = ten[o + u * P];
+                cp[u] = RK == 0 ? T(0) : cpy[o + u * P];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (RK == 0) cpy[o + u * P] = phi[u];
+                val[o + u * P] = rk_update<T, RK>(phi[u], tn[u], cp[u], dt);
+                if (zero_tend) ten[o + u * P] = T(0);
+            }
+        }
+        for (; k < k1; ++k) {
+            const long long o = (long long)k * P + col;
             const T phi = val[o];
             const T tn = ten[o];
-            T out;
-            if (rk_step == 0) {
-                cpy[o] = phi;
-                out = phi + tn * (T)dt;
-            } else if (rk_step == 1) {
-                out = T(0.75) * cpy[o] + T(0.25) * (phi + tn * (T)dt);
-            } else {
-                out = T(1.0 / 3.0) * cpy[o] + T(2.0 / 3.0) * (phi + tn * (T)dt);
-            }
-            val[o] = out;
+            const T cp = RK == 0 ? T(0) : cpy[o];
+            if (RK == 0) cpy[o] = phi;
+            val[o] = rk_update<T, RK>(phi, tn, cp, dt);
             if (zero_tend) ten[o] = T(0);
         }
         if (zero_tend) {
@@ -150,7 +172,7 @@ static int pack(int nv, void *const *values, void *const *tend, void *const *cop
 
 extern "C" int pycles_rk3_stage(const pycles_dims *dims, int nv, void *const *values, void *const *tend,
                                 void *const *copies, const double *bc_type, int rk_step, double dt,
-                                int precision, void *stream) {
+                                int precision, int zero_tendencies, void *stream) {
     int rc = pycles_check_dims(dims);
     if (rc) return rc;
     if (rk_step < 0 || rk_step > 2 || !tend || !copies || !values || !bc_type) {
@@ -161,16 +183,24 @@ extern "C" int pycles_rk3_stage(const pycles_dims *dims, int nv, void *const *va
     if ((rc = pack(nv, values, tend, copies, bc_type, &pk))) return rc;
     Geo g = make_geo(dims);
     const int B = 256;
-    const unsigned grid = (unsigned)((g.plane + B - 1) / B);
+    const dim3 grid((unsigned)((g.plane + B - 1) / B), nv);
     hipStream_t st = (hipStream_t)stream;
-    if (precision == PYCLES_F64)
-        hipLaunchKernelGGL(rk3_kernel<double>, dim3(grid), dim3(B), 0, st, g, nv, pk, rk_step, dt, 1);
-    else if (precision == PYCLES_F32)
-        hipLaunchKernelGGL(rk3_kernel<float>, dim3(grid), dim3(B), 0, st, g, nv, pk, rk_step, dt, 1);
-    else {
+    if (precision != PYCLES_F64 && precision != PYCLES_F32) {
         pycles_set_error("bad precision %d", precision);
         return PYCLES_ERR_ARG;
     }
+    const int zt = zero_tendencies ? 1 : 0;
+#define RK_LAUNCH(T, R) hipLaunchKernelGGL((rk3_kernel<T, R>), grid, dim3(B), 0, st, g, pk, dt, zt)
+    if (precision == PYCLES_F64) {
+        if (rk_step == 0) RK_LAUNCH(double, 0);
+        else if (rk_step == 1) RK_LAUNCH(double, 1);
+        else RK_LAUNCH(double, 2);
+    } else {
+        if (rk_step == 0) RK_LAUNCH(float, 0);
+        else if (rk_step == 1) RK_LAUNCH(float, 1);
+        else RK_LAUNCH(float, 2);
+    }
+#undef RK_LAUNCH
     PYCLES_CHECK_LAUNCH();
     return PYCLES_OK;
 }

@@ -264,3 +264,21 @@ def test_level_mean_and_cfl_entry_points(gpu):
     L.orc_cfl_max.argtypes = [C.POINTER(O.OrcDims), O._dp, O._dp, O._dp, C.c_double]
     want = L.orc_cfl_max(C.byref(osim.d), O.ptr(osim.pv("u")), O.ptr(osim.pv("v")), O.ptr(osim.pv("w")), 0.7)
     assert float(out.item()) == want
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_written_tendencies_match_zeroed(gpu, precision):
+    """Writing the tendencies in the first operator of a stage instead of
+    zeroing them in the RK update gives identical bits."""
+    import json
+    nl, Gr, RS, Th, fields = host_case(32, 16, 24, seed=13)
+    nl = json.loads(json.dumps(nl))
+    nl["mi355x"]["precision"] = precision
+    out = []
+    for fast in (True, False):
+        sim = _gpu_sim(nl, Gr, RS, Th, fields)
+        sim.write_tendencies = fast
+        sim.run(max_steps=2)
+        torch.cuda.synchronize()
+        out.append(sim.PV.values.clone())
+    assert torch.equal(out[0], out[1])
