@@ -1,0 +1,93 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point that
+include/pycles_amd.h declares, and rejects bad arguments before any device
+call (argument checks run on the host, so these calls never touch a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pycles_amd.h")
+LIB = os.path.join(ROOT, "pycles_amd", "libpycles_amd.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="libpycles_amd.so not built")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|void|char)\s*\*?\s*(pycles_\w+)\s*\(", text, flags=re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return C.CDLL(LIB)
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for n in ("pycles_advection", "pycles_thermo_sa", "pycles_thermo_dry", "pycles_rk3_stage",
+              "pycles_pressure_plan_create", "pycles_pressure_update", "pycles_pressure_plan_destroy"):
+        assert n in names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+class Dims(C.Structure):
+    _fields_ = [("n", C.c_int64 * 3), ("gw", C.c_int64), ("dx", C.c_double * 3), ("dxi", C.c_double * 3)]
+
+
+class Ref(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("rho0", "rho0_half", "alpha0", "alpha0_half", "p0", "p0_half")]
+
+
+def _dims(n=(16, 16, 8), gw=3):
+    d = Dims()
+    for i in range(3):
+        d.n[i] = n[i]
+        d.dx[i] = 10.0
+        d.dxi[i] = 0.1
+    d.gw = gw
+    return d
+
+
+def test_version_and_error_text(lib):
+    lib.pycles_version.restype = C.c_char_p
+    lib.pycles_last_error.restype = C.c_char_p
+    assert lib.pycles_version().decode()
+    assert isinstance(lib.pycles_last_error(), bytes)
+
+
+def _advection(lib, dims, so, mo, nscal=1, precision=0):
+    fake = C.c_void_p(0x1000)      # never dereferenced: the checks fail first
+    ref = Ref(*([fake.value] * 6))
+    vel = (C.c_void_p * 3)(fake.value, fake.value, fake.value)
+    tv = (C.c_void_p * 3)(fake.value, fake.value, fake.value)
+    sc = (C.c_void_p * 2)(fake.value, fake.value)
+    ts = (C.c_void_p * 2)(fake.value, fake.value)
+    lib.pycles_advection.restype = C.c_int
+    return lib.pycles_advection(C.byref(dims), C.byref(ref), vel, tv, nscal, sc, ts, None, so, mo,
+                                precision, 0, None)
+
+
+def test_advection_rejects_bad_arguments(lib):
+    lib.pycles_last_error.restype = C.c_char_p
+    assert _advection(lib, _dims(), 6, 5) == -3           # order 6: unsupported
+    assert b"order" in lib.pycles_last_error()
+    assert _advection(lib, _dims(gw=3), 9, 5) == -1       # WENO9 needs gw >= 5
+    assert _advection(lib, _dims(n=(5, 16, 8)), 5, 5) == -1   # nx < 2r
+    assert _advection(lib, _dims(), 5, 5, nscal=3) == -1
+    assert _advection(lib, _dims(n=(16, 16, 0)), 5, 5) == -1
+    assert _advection(lib, _dims(), 5, 5, precision=7) == -1
+
+
+def test_pressure_plan_rejects_unsupported_sizes(lib):
+    lib.pycles_pressure_plan_create.restype = C.c_int
+    rho = (C.c_double * 64)(*([1.0] * 64))
+    plan = C.c_void_p()
+    rc = lib.pycles_pressure_plan_create(C.byref(_dims(n=(12, 16, 8))), rho, rho, 0, C.byref(plan))
+    assert rc != 0 and not plan.value

@@ -1,0 +1,100 @@
+"""Host-side logic on the CPU: the shared thermodynamic inputs (golden
+fixture, hydrostatic balance, uniform entropy of the reference state), the
+layout conversions, the counter-based initial-noise generator and the
+divergence-free initial velocities."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pycles_amd import cases
+from pycles_amd import parameters as P
+from pycles_amd.Grid import Grid
+from pycles_amd.ReferenceState import ReferenceState
+from pycles_amd.ThermodynamicsSA import ThermodynamicsFactory
+from tests.helpers import host_case
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "thermo_profiles.npz")
+
+
+def test_thermo_inputs_match_golden():
+    from tests.golden.make_golden import compute
+    want = np.load(GOLDEN)
+    got = compute()
+    assert set(want.files) == set(got)
+    for k in want.files:
+        np.testing.assert_allclose(got[k], want[k], rtol=1e-12, atol=0, err_msg=k)
+
+
+@pytest.mark.parametrize("moist", [True, False])
+def test_reference_state_is_hydrostatic_and_isentropic(moist):
+    """d ln p0/dz = -g / (Rd T (1 - qt + qv/eps_v)) (ReferenceState.pyx:54-56)
+    and the saturation-adjusted entropy is uniform (:112-121, tightened)."""
+    nl, Gr, RS, Th, _ = host_case(4, 4, 64, moist=moist, dz=25.0)
+    gw = Gr.dims.gw
+    z = Gr.z_half[gw:-gw]
+    lp = np.log(RS.p0_half[gw:-gw])
+    dlp = np.gradient(lp, z)
+    lt = O.Lookup(Th.CC.T, Th.CC.pv) if moist else None
+    for k in range(2, len(z) - 2):
+        p0 = RS.p0_half[gw + k]
+        if moist:
+            T, qv, ql, qi, _ = O.eos_sa(lt, p0, RS.sg, RS.qtg)
+            s = O.lib().orc_entropy_sa(p0, T, RS.qtg, ql, qi)
+            assert abs(s - RS.sg) / RS.sg < 1e-5
+        else:
+            T = O.lib().orc_eos_dry(p0, RS.sg)
+            ql = qi = 0.0
+        rhs = -P.g / (P.Rd * T * (1.0 - RS.qtg + P.eps_vi * (RS.qtg - ql - qi)))
+        assert dlp[k] == pytest.approx(rhs, rel=2e-4)
+        # alpha0_half is the specific volume of that state
+        qv = RS.qtg - ql - qi
+        assert RS.alpha0_half[gw + k] == pytest.approx(O.lib().orc_alpha(p0, T, RS.qtg, qv), rel=1e-10)
+
+
+def test_layout_round_trip():
+    nl = cases.namelist(6, 5, 4, 3, 1.0, 1.0, 1.0, False)
+    Gr = Grid(nl)
+    d = Gr.dims
+    rng = np.random.default_rng(1)
+    dev = rng.normal(size=(d.nzg, d.n[1], d.n[0]))
+    ref = Gr.from_device_layout(dev)
+    assert ref.shape == tuple(d.nlg)
+    assert np.array_equal(Gr.to_device_layout(ref), dev)
+    gw = d.gw
+    # periodic images in x and y
+    assert np.array_equal(ref[:gw], ref[d.n[0]:d.n[0] + gw])
+    assert np.array_equal(ref[:, -gw:], ref[:, gw:2 * gw])
+    assert ref[gw + 2, gw + 1, 5] == dev[5, 1, 2]
+
+
+def test_noise_generator():
+    shape = (8, 16, 32)
+    a = cases.uniform(7, 3, shape)
+    assert np.array_equal(a, cases.uniform(7, 3, shape))
+    assert a.min() >= 0.0 and a.max() < 1.0
+    assert abs(a.mean() - 0.5) < 0.02 and abs(a.var() - 1.0 / 12.0) < 0.01
+    b = cases.uniform(7, 4, shape)
+    c = cases.uniform(8, 3, shape)
+    assert abs(np.corrcoef(a.ravel(), b.ravel())[0, 1]) < 0.05
+    assert abs(np.corrcoef(a.ravel(), c.ravel())[0, 1]) < 0.05
+    g = cases.normal(7, 1, (64, 64, 16))
+    assert abs(g.mean()) < 0.02 and abs(g.std() - 1.0) < 0.02
+    # counter-based: a sub-block is a prefix of the same stream
+    assert np.array_equal(cases.uniform(7, 3, (4096,))[:100], cases.uniform(7, 3, (100,)))
+
+
+@pytest.mark.parametrize("moist", [True, False])
+def test_initial_velocities_are_divergence_free(moist):
+    nl, Gr, RS, Th, fields = host_case(16, 16, 12, moist=moist, seed=4, sigma=1.0)
+    d = Gr.dims
+    dims = O.make_dims(d.n[0], d.n[1], d.n[2], d.gw, *d.dx)
+    ref = O.Ref(RS.p0, RS.p0_half, RS.alpha0, RS.alpha0_half, RS.rho0, RS.rho0_half)
+    sim = O.OracleSimulation(dims, ref, False, None)
+    for n in ("u", "v", "w"):
+        sim.pv(n)[...] = Gr.from_device_layout(fields[n])
+    gw = d.gw
+    div = sim.divergence()[gw:-gw, gw:-gw, gw:-gw]
+    scale = np.abs(Gr.from_device_layout(fields["u"])).max() / d.dx[0]
+    assert np.abs(div).max() < 1e-12 * scale

@@ -1,0 +1,458 @@
+"""Pins of the CPU oracle by what the mathematics fixes, independent of any
+implementation: WENO coefficients derived from their definition, polynomial
+exactness and convergence order, conservation and upwinding of the flux
+divergence, the projection against a manufactured discrete solution, the
+reference's TDMA example, the DFT, the SSP-RK3 amplification factor, the
+Clausius-Clapeyron table, and the saturation-adjustment inversion.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pycles_amd import parameters as P
+from tests.weno_theory import (BETA_SCALE, linear_reconstruction, weno_coefficients,
+                               weno_from_definition)
+
+ORDERS = [5, 7, 9]
+
+
+# --------------------------------------------------------------------------
+# WENO interpolation
+
+def test_weno_linear_weights_and_beta_normalisation():
+    """The derived optimal weights are the textbook ones and the scaled
+    smoothness forms have integer coefficients (Balsara & Shu's normalisation)."""
+    _, d3, _ = weno_coefficients(3)
+    _, d4, _ = weno_coefficients(4)
+    _, d5, _ = weno_coefficients(5)
+    assert [float(x) for x in d3] == [0.1, 0.6, 0.3]
+    assert sum(d4) == 1 and sum(d5) == 1
+    for r in (4, 5):
+        _, _, Q = weno_coefficients(r)
+        s = BETA_SCALE[r]
+        for q in Q:
+            for a in range(r):
+                assert (s * q[a][a]).denominator == 1
+                for b in range(a + 1, r):
+                    assert (2 * s * q[a][b]).denominator == 1
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_weno_matches_definition(order):
+    rng = np.random.default_rng(order)
+    width = order
+    worst = 0.0
+    for n in range(3000):
+        kind = n % 4
+        if kind == 0:
+            q = rng.normal(size=width)
+        elif kind == 1:
+            q = np.cumsum(rng.normal(size=width)) * rng.uniform(0.01, 100)
+        elif kind == 2:
+            q = np.where(np.arange(width) < rng.integers(0, width + 1), 1.0, 0.0) + 0.01 * rng.normal(size=width)
+        else:
+            q = np.sin(0.4 * np.arange(width) + rng.uniform(0, 6))
+        a = O.weno(order, q)
+        b = weno_from_definition(q, order)
+        worst = max(worst, abs(a - b) / max(np.max(np.abs(q)), 1e-300))
+    # rounding in the 7-digit smoothness coefficients of WENO9 reaches ~2e-12;
+    # a unit error in any one integer coefficient moves the result by > 1e-4
+    assert worst < 1e-10, worst
+
+
+def _cell_averages_of_poly(coef, centres, h=1.0):
+    """Exact cell averages over [x - h/2, x + h/2] of sum coef[e] x^e."""
+    out = []
+    for x in centres:
+        lo, hi = x - h / 2, x + h / 2
+        out.append(sum(c * (hi ** (e + 1) - lo ** (e + 1)) / (e + 1) for e, c in enumerate(coef)) / h)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_weno_exact_for_low_degree(order):
+    """All candidates reproduce a polynomial of degree <= r-1, so WENO returns
+    its value at the interface whatever the nonlinear weights are."""
+    r = (order + 1) // 2
+    rng = np.random.default_rng(40 + order)
+    centres = np.arange(-(r - 1), r, dtype=np.float64)
+    for _ in range(200):
+        coef = rng.normal(size=r)
+        q = _cell_averages_of_poly(coef, centres)
+        exact = sum(c * 0.5 ** e for e, c in enumerate(coef))
+        assert abs(O.weno(order, q) - exact) <= 1e-12 * max(1.0, np.max(np.abs(q)))
+        # mirror: the reversed stencil gives the value at the left face of the
+        # centre cell (how the flux forms phi^- from the downwind cell)
+        exact_m = sum(c * (-0.5) ** e for e, c in enumerate(coef))
+        assert abs(O.weno(order, q[::-1]) - exact_m) <= 1e-12 * max(1.0, np.max(np.abs(q)))
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_weno_offset_invariance(order):
+    rng = np.random.default_rng(50 + order)
+    for _ in range(500):
+        q = rng.normal(size=order)
+        c = rng.uniform(-50, 50)
+        assert abs((O.weno(order, q + c) - c) - O.weno(order, q)) <= 1e-12 * (abs(c) + 1)
+
+
+@pytest.mark.parametrize("order,hs", [(5, (0.2, 0.1)), (7, (0.4, 0.2)), (9, (0.6, 0.3))])
+def test_weno_convergence_order(order, hs):
+    """Smooth monotone data (no critical point): the error at the interface
+    falls as h^(2r-1) and WENO approaches the linear reconstruction."""
+    r = (order + 1) // 2
+    errs = []
+    for h in hs:
+        x = 0.3 + h * np.arange(-(r - 1), r)
+        q = (np.exp(x + h / 2) - np.exp(x - h / 2)) / h   # cell averages of exp
+        errs.append(abs(O.weno(order, q) - np.exp(0.3 + h / 2)))
+        assert abs(O.weno(order, q) - linear_reconstruction(q, order)) < 10 * errs[-1] + 1e-15
+    rate = np.log2(errs[0] / errs[1])
+    assert rate > 2 * r - 1 - 0.5, (errs, rate)
+
+
+@pytest.mark.parametrize("order,name", [(5, 4), (7, 6), (9, 8)])
+def test_centred_exact(order, name):
+    """interp_4/6/8: the interface value between the two middle cells from n
+    cell averages, exact for polynomials of degree < n."""
+    n = name
+    rng = np.random.default_rng(60 + n)
+    x = np.arange(n) - (n // 2 - 1.0)        # interface at 0.5
+    for _ in range(100):
+        coef = rng.normal(size=n)
+        q = _cell_averages_of_poly(coef, x)
+        exact = np.polynomial.polynomial.polyval(0.5, coef)
+        assert abs(O.centred(order, q) - exact) <= 1e-12 * max(1.0, np.max(np.abs(q)))
+
+
+# --------------------------------------------------------------------------
+# Flux divergence
+
+def _dry_sim(nx, ny, nz, gw, dx, rho0, rho0_half, order=5):
+    d = O.make_dims(nx, ny, nz, gw, *dx)
+    ref = O.Ref(np.full(d.nlg[2], 1e5), np.full(d.nlg[2], 1e5), 1.0 / rho0, 1.0 / rho0_half, rho0, rho0_half)
+    return O.OracleSimulation(d, ref, False, None, scalar_order=order, momentum_order=order), d
+
+
+@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("direction", [0, 1, 2])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_advection_exact_for_polynomials(order, direction, sign):
+    """FD-WENO flux difference of point values of a polynomial of degree <= r-1
+    is the exact derivative: uniform U gives tendency -U phi'(x_i)."""
+    r = (order + 1) // 2
+    gw = r
+    n = (10, 9, 8)
+    dx = (30.0, 45.0, 20.0)
+    nlg2 = n[2] + 2 * gw
+    rho = np.ones(nlg2)
+    sim, d = _dry_sim(*n, gw, dx, rho, rho, order)
+    rng = np.random.default_rng(order * 10 + direction)
+    coef = rng.normal(size=r) / np.array([max(1.0, 5.0 ** e) for e in range(r)])
+    idx = np.indices(tuple(d.nlg), dtype=np.float64)[direction]
+    xs = (idx - gw) * dx[direction] / 100.0
+    sim.pv("s")[...] = np.polynomial.polynomial.polyval(xs, coef)
+    U = 3.0 * sign
+    vel = sim.pv(("u", "v", "w")[direction])
+    vel[...] = U
+    sim.scalar_advection_update()
+    deriv = np.polynomial.polynomial.polyval(xs, np.polynomial.polynomial.polyder(coef)) / 100.0
+    sl = (slice(gw, -gw),) * 3
+    got = sim.pt("s")[sl]
+    want = -U * deriv[sl]
+    assert np.max(np.abs(got - want)) <= 1e-11 * max(1.0, np.max(np.abs(want))), np.max(np.abs(got - want))
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_advection_conserves(order):
+    """Periodic x/y and zero w at the bottom and top faces: the flux differences
+    telescope, so the mass-weighted sum of the scalar tendency vanishes."""
+    gw = (order + 1) // 2
+    n = (12, 10, 9)
+    rng = np.random.default_rng(70 + order)
+    nlg2 = n[2] + 2 * gw
+    rho0 = rng.uniform(0.7, 1.2, nlg2)
+    rho0_half = rng.uniform(0.7, 1.2, nlg2)
+    sim, d = _dry_sim(*n, gw, (40.0, 50.0, 25.0), rho0, rho0_half, order)
+    for name, s in (("u", 4.0), ("v", 3.0), ("w", 1.0), ("s", 50.0)):
+        sim.pv(name)[...] = rng.normal(size=tuple(d.nlg)) * s
+    sim.pv("w")[:, :, gw - 1] = 0.0
+    sim.pv("w")[:, :, d.nlg[2] - gw - 1] = 0.0
+    sim.update_all_bcs()
+    sim.pv("w")[:, :, d.nlg[2] - gw - 1] = 0.0
+    sim.scalar_advection_update()
+    t = sim.pt("s")[gw:-gw, gw:-gw, gw:-gw]
+    mass = t * rho0_half[gw:-gw][None, None, :]
+    assert abs(mass.sum()) <= 1e-13 * np.abs(mass).sum()
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_advection_uniform_state_is_steady(order):
+    gw = (order + 1) // 2
+    n = (8, 8, 6)
+    rng = np.random.default_rng(80)
+    rho0 = rng.uniform(0.7, 1.2, n[2] + 2 * gw)
+    sim, d = _dry_sim(*n, gw, (40.0, 50.0, 25.0), rho0, rho0, order)
+    sim.pv("u")[...] = 2.5
+    sim.pv("v")[...] = -1.5
+    sim.pv("s")[...] = 301.25
+    sim.scalar_advection_update()
+    assert np.all(sim.pt("s") == 0.0)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_advection_upwinds_by_sign(order):
+    """A step in x: the face value follows the upwind side of the step."""
+    gw = (order + 1) // 2
+    n = (16, 6, 4)
+    rho = np.ones(n[2] + 2 * gw)
+    for U, want in ((2.0, 0.0), (-2.0, 1.0)):
+        sim, d = _dry_sim(*n, gw, (10.0, 10.0, 10.0), rho, rho, order)
+        i0 = gw + 7
+        s = sim.pv("s")
+        s[...] = 0.0
+        s[i0 + 1:, :, :] = 1.0
+        sim.pv("u")[...] = U
+        flux = np.zeros(tuple(d.nlg))
+        O.lib().orc_scalar_flux(C.byref(d), O.ptr(rho), O.ptr(rho), O.ptr(sim.pv("u")), O.ptr(s),
+                                O.ptr(flux), 0, order)
+        face = flux[i0, gw + 1, gw + 1] / U
+        assert abs(face - want) < 1e-6, (U, face)
+
+
+# --------------------------------------------------------------------------
+# Pressure projection
+
+def _pressure_case(seed=0, n=(16, 8, 12), gw=3, dx=(50.0, 40.0, 20.0)):
+    rng = np.random.default_rng(seed)
+    nlg2 = n[2] + 2 * gw
+    rho0 = rng.uniform(0.7, 1.2, nlg2)
+    rho0_half = rng.uniform(0.7, 1.2, nlg2)
+    sim, d = _dry_sim(*n, gw, dx, rho0, rho0_half)
+    return sim, d, rng
+
+
+def test_pressure_tdma_known_answer():
+    """SparseSolvers.pyx:49-60: the reference's own example solves to [2, 3, 5, 7]."""
+    x = O.tdma([5, 5, 10, 23], [0, -1, -1, -1], [4, 4, 4, 4], [-1, -1, -1, 0])
+    assert np.allclose(x, [2, 3, 5, 7], rtol=0, atol=1e-14)
+
+
+def test_pressure_tdma_random():
+    rng = np.random.default_rng(3)
+    n = 40
+    a, c = rng.normal(size=n), rng.normal(size=n)
+    a[0] = c[-1] = 0.0
+    b = 4.0 + np.abs(a) + np.abs(c)
+    rhs = rng.normal(size=n)
+    A = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+    assert np.allclose(O.tdma(rhs, a, b, c), np.linalg.solve(A, rhs), rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 12, 16, 64])
+def test_fft_is_the_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    m = np.arange(n)
+    F = np.exp(-2j * np.pi * np.outer(m, m) / n)
+    X = O.fft(x)
+    assert np.allclose(X, F @ x, rtol=0, atol=1e-12 * n)
+    assert np.allclose(O.fft(X, inverse=True), x, rtol=0, atol=1e-13 * n)
+    assert abs(np.sum(np.abs(X) ** 2) / n - np.sum(np.abs(x) ** 2)) < 1e-12 * n * np.sum(np.abs(x) ** 2)
+
+
+def _apply_gradient(d, p, u, v, w):
+    """u -= G p on [0, nlg-1)^3 as the projection's correction does (read via
+    the oracle with zero divergence would be circular, so spelled here as the
+    forward difference that the discrete divergence is the adjoint of)."""
+    u[:-1, :-1, :-1] += (p[1:, :-1, :-1] - p[:-1, :-1, :-1]) / d.dx[0]
+    v[:-1, :-1, :-1] += (p[:-1, 1:, :-1] - p[:-1, :-1, :-1]) / d.dx[1]
+    w[:-1, :-1, :-1] += (p[:-1, :-1, 1:] - p[:-1, :-1, :-1]) / d.dx[2]
+
+
+def test_pressure_manufactured_solution():
+    """u = G p* for a horizontally mean-free p*: the discrete elliptic operator
+    D rho G (modified wavenumbers in x, y; the tridiagonal in z) must return
+    p* and leave a zero velocity after the correction."""
+    sim, d, rng = _pressure_case(seed=5)
+    gw = d.gw
+    shape = tuple(d.nlg)
+    i, j, k = np.indices(shape, dtype=np.float64)
+    nx, ny, nz = d.n[0], d.n[1], d.n[2]
+    fz = rng.normal(size=shape[2])
+    gz = rng.normal(size=shape[2])
+    pstar = (np.cos(2 * np.pi * (i - gw) / nx) * fz[None, None, :]
+             + np.sin(2 * np.pi * 3 * (i - gw) / nx + 0.4) * np.cos(2 * np.pi * 2 * (j - gw) / ny) * gz[None, None, :]
+             + 0.5 * np.sin(2 * np.pi * (j - gw) / ny) * fz[None, None, ::-1])
+    lib = O.lib()
+    lib.orc_update_bcs(C.byref(d), O.ptr(pstar), C.c_double(1.0))
+    for n in ("u", "v", "w"):
+        sim.pv(n)[...] = 0.0
+    _apply_gradient(d, pstar, sim.pv("u"), sim.pv("v"), sim.pv("w"))
+    for n, bc in (("u", 1.0), ("v", 1.0), ("w", -1.0)):
+        lib.orc_update_bcs(C.byref(d), O.ptr(sim.pv(n)), C.c_double(bc))
+    scale = max(np.abs(sim.pv(n)).max() for n in ("u", "v", "w"))
+    sim.pressure_update()
+    sl = (slice(gw, -gw),) * 3
+    p = sim.dv["dynamic_pressure"]
+    assert np.max(np.abs(p[sl] - pstar[sl])) <= 1e-10 * np.abs(pstar[sl]).max()
+    for n in ("u", "v", "w"):
+        assert np.max(np.abs(sim.pv(n)[sl])) <= 1e-10 * scale, n
+
+
+def test_pressure_projects_random_field():
+    sim, d, rng = _pressure_case(seed=6, n=(16, 16, 10))
+    gw = d.gw
+    for n, s in (("u", 3.0), ("v", 2.0), ("w", 1.0)):
+        sim.pv(n)[...] = rng.normal(size=tuple(d.nlg)) * s
+    sim.update_all_bcs()
+    sl = (slice(gw, -gw),) * 3
+    div0 = np.abs(sim.divergence()[sl]).max()
+    sim.pressure_update()
+    div1 = np.abs(sim.divergence()[sl]).max()
+    assert div1 <= 1e-12 * div0
+    p = sim.dv["dynamic_pressure"][sl]
+    assert np.abs(p.mean(axis=(0, 1))).max() <= 1e-12 * np.abs(p).max()
+
+
+# --------------------------------------------------------------------------
+# Time stepping, boundaries, CFL
+
+def test_rk3_amplification_factor():
+    """Three stages on dy/dt = lambda y give 1 + z + z^2/2 + z^3/6."""
+    lam = np.array([-0.3, 0.7, -2.0, 1.3])
+    dt = 0.4
+    y = np.array([1.0, 2.0, -0.5, 3.0])
+    vals = y.copy()
+    tend = np.zeros_like(vals)
+    copy = np.zeros_like(vals)
+    for stage in range(3):
+        tend[...] = lam * vals
+        O.lib().orc_rk3_update(vals.size, stage, dt, O.ptr(vals), O.ptr(tend), O.ptr(copy))
+        assert np.all(tend == 0.0)
+    z = lam * dt
+    assert np.allclose(vals, y * (1 + z + z * z / 2 + z ** 3 / 6), rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("bc", [1.0, -1.0])
+def test_boundary_identities(bc):
+    gw = 3
+    d = O.make_dims(5, 4, 6, gw, 1.0, 1.0, 1.0)
+    rng = np.random.default_rng(9)
+    v = rng.normal(size=tuple(d.nlg))
+    O.lib().orc_update_bcs(C.byref(d), O.ptr(v), C.c_double(bc))
+    n0, n1, n2 = d.n[0], d.n[1], d.n[2]
+    assert np.array_equal(v[:gw], v[n0:n0 + gw]) and np.array_equal(v[-gw:], v[gw:2 * gw])
+    assert np.array_equal(v[:, :gw], v[:, n1:n1 + gw]) and np.array_equal(v[:, -gw:], v[:, gw:2 * gw])
+    top = d.nlg[2] - gw
+    if bc == 1.0:
+        for k in range(gw):
+            assert np.array_equal(v[:, :, gw - 1 - k], v[:, :, gw + k])
+            assert np.array_equal(v[:, :, top + k], v[:, :, top - 1 - k])
+    else:
+        assert np.all(v[:, :, gw - 1] == 0.0) and np.all(v[:, :, top] == 0.0)
+        for k in range(1, gw):
+            assert np.array_equal(v[:, :, gw - 1 - k], -v[:, :, gw - 1 + k])
+            assert np.array_equal(v[:, :, top + k], -v[:, :, top - k])
+
+
+def test_cfl_and_horizontal_mean():
+    gw = 3
+    d = O.make_dims(6, 5, 4, gw, 20.0, 30.0, 10.0)
+    rng = np.random.default_rng(10)
+    u, v, w = (rng.normal(size=tuple(d.nlg)) for _ in range(3))
+    L = O.lib()
+    L.orc_cfl_max.argtypes = [C.POINTER(O.OrcDims), O._dp, O._dp, O._dp, C.c_double]
+    L.orc_cfl_max.restype = C.c_double
+    dt = 0.7
+    sl = (slice(gw, -gw),) * 3
+    want = np.max(dt * (np.abs(u[sl]) / 20.0 + np.abs(v[sl]) / 30.0 + np.abs(w[sl]) / 10.0))
+    got = L.orc_cfl_max(C.byref(d), O.ptr(u), O.ptr(v), O.ptr(w), dt)
+    assert abs(got - 1e-11 - want) <= 1e-14 * want
+    mean = np.zeros(d.nlg[2])
+    L.orc_horizontal_mean(C.byref(d), O.ptr(u), O.ptr(mean))
+    assert np.allclose(mean, u[gw:-gw, gw:-gw, :].mean(axis=(0, 1)), rtol=0, atol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# Thermodynamics
+
+@pytest.fixture(scope="module")
+def cc_table():
+    from pycles_amd.Thermodynamics import ClausiusClapeyron, LatentHeat
+    cc = ClausiusClapeyron().initialize(None, LatentHeat())
+    return cc, O.Lookup(cc.T, cc.pv)
+
+
+def test_clausius_clapeyron_table(cc_table):
+    """Triple point pv*(273.16) = 611.7 Pa and d ln pv*/dT = L(T)/(Rv T^2)."""
+    cc, lt = cc_table
+    assert abs(lt(P.Tt) - P.pv_star_t) / P.pv_star_t < 1e-4
+    T = cc.T[200:400]
+    h = 1e-3
+    dlnp = (np.log([lt(t + h) for t in T]) - np.log([lt(t - h) for t in T])) / (2 * h)
+    L = np.array([O.lib().orc_latent_heat_variable(t) for t in T])
+    # piecewise-linear interpolation: compare at nodes with a one-sided-safe tolerance
+    assert np.max(np.abs(dlnp / (L / (P.Rv * T * T)) - 1.0)) < 5e-3
+    # table nodes are reproduced exactly by the lookup
+    for i in (3, 100, 300, 500):
+        assert lt(cc.T[i]) == pytest.approx(cc.pv[i], rel=1e-13)
+
+
+def test_entropy_derivatives():
+    """ds/dT = cpm/T and ds/dln p0 = -((1-qt) Rd + qt Rv) on the unsaturated
+    branch; dry air at (p_tilde, T_tilde) has sd_tilde.  At qt = 0 the moist
+    form multiplies log(pv = 0) by zero and is NaN, as the reference's is."""
+    L = O.lib()
+    assert L.orc_entropy_dry(P.p_tilde, P.T_tilde) == pytest.approx(P.sd_tilde, rel=1e-15)
+    assert np.isnan(L.orc_entropy_sa(P.p_tilde, P.T_tilde, 0.0, 0.0, 0.0))
+    for p0, T, qt in ((9e4, 290.0, 0.005), (7e4, 270.0, 0.002), (1e5, 300.0, 1e-6)):
+        h = 1e-4
+        dsdT = (L.orc_entropy_sa(p0, T + h, qt, 0.0, 0.0) - L.orc_entropy_sa(p0, T - h, qt, 0.0, 0.0)) / (2 * h)
+        cpm = (1 - qt) * P.cpd + qt * P.cpv
+        assert dsdT == pytest.approx(cpm / T, rel=1e-7)
+        e = 1e-5
+        dsdlnp = (L.orc_entropy_sa(p0 * np.exp(e), T, qt, 0.0, 0.0)
+                  - L.orc_entropy_sa(p0 * np.exp(-e), T, qt, 0.0, 0.0)) / (2 * e)
+        assert dsdlnp == pytest.approx(-((1 - qt) * P.Rd + qt * P.Rv), rel=1e-7)
+
+
+def test_dry_eos_round_trip():
+    L = O.lib()
+    for pd in (1e5, 8e4, 5e4):
+        for T in (220.0, 288.0, 310.0):
+            s = L.orc_entropy_dry(pd, T)
+            assert L.orc_eos_dry(pd, s) == pytest.approx(T, rel=1e-13)
+
+
+def test_sa_eos_unsaturated_round_trip(cc_table):
+    _, lt = cc_table
+    L = O.lib()
+    for p0, T, qt in ((9.5e4, 295.0, 0.008), (8e4, 280.0, 0.004), (6e4, 260.0, 0.001)):
+        s = L.orc_entropy_sa(p0, T, qt, 0.0, 0.0)
+        T2, qv, ql, qi, it = O.eos_sa(lt, p0, s, qt)
+        assert it == 0 and ql == 0.0 and qi == 0.0 and qv == qt
+        assert T2 == pytest.approx(T, rel=1e-13)
+
+
+def test_sa_eos_saturated_round_trip(cc_table):
+    """Build a saturated state (qv from Dalton's law at pv = pv*(T)), take its
+    entropy, invert: T is recovered to the secant's 1e-3 K stop, the vapour is
+    saturated at the new T, and qv + ql = qt."""
+    _, lt = cc_table
+    L = O.lib()
+    for p0, T, qt in ((9e4, 290.0, 0.02), (8e4, 283.0, 0.016), (9.8e4, 300.0, 0.03), (7e4, 275.0, 0.012)):
+        pv = lt(T)
+        qv = P.eps_v * (1.0 - qt) * pv / (p0 - pv)
+        assert qv < qt
+        s = L.orc_entropy_sa(p0, T, qt, qt - qv, 0.0)
+        T2, qv2, ql2, qi2, it = O.eos_sa(lt, p0, s, qt)
+        assert it > 0 and abs(T2 - T) < 1e-3
+        assert ql2 > 0.0 and qi2 == 0.0 and qv2 + ql2 == pytest.approx(qt, rel=1e-15)
+        pv2 = p0 * qv2 / (P.eps_v * (1.0 - qt) + qv2)      # Dalton's law
+        # qv is saturated at the last evaluated iterate, which the final secant
+        # update moves by < 1e-3 K (d ln pv*/dT < 0.08 / K)
+        assert pv2 == pytest.approx(lt(T2), rel=1e-4)
+        assert abs(L.orc_entropy_sa(p0, T2, qt, ql2, 0.0) - s) < 1e-3 * 10.0
