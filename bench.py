@@ -1,18 +1,22 @@
 """Benchmark: Mcell-updates/s per SSP-RK3 step of the dynamical core.
 
 Workload: the moist 512x512x256 configuration (dx=dy=50 m, dz=20 m, WENO5
-scalars and momentum, saturation adjustment, FFT + tridiagonal pressure,
-dt=1 s), seeded synthetic state.  A step is the full SSP-RK3 step (three
-stages of thermodynamics, advection, RK update with boundaries, pressure
-projection; CFL once).  Inputs are resident in HBM before timing starts.
+scalars and momentum, saturation adjustment with a saturated cloud layer,
+FFT + tridiagonal pressure, dt=1 s), seeded synthetic state.  A step is the
+full SSP-RK3 step (three stages of thermodynamics, scalar and momentum
+advection, RK update with boundaries, pressure projection; CFL once).  Inputs
+are resident in HBM before timing starts.
 
-The same initial state is run in plain FP64 and in the FP32 form; ``value``
-is the FP32-form throughput, ``fp64`` the FP64 throughput, ``speedup`` their
-ratio, and ``fp32_vs_fp64`` the max / RMS relative difference per field after
-warmup+steps steps.  ``roofline`` is for the dominant operator of the FP32
-form, timed with HIP events on the working stream during the timed steps;
-``cpu_baseline`` times the CPU oracle (FP64 C restatement, OpenMP) on a
-bounded sample on the host cores.
+Forms, all from the same initial state:
+  value      FP32 + learned guess (the network is trained here first, on the
+             GPU, outside the timed region)
+  fp32_plain FP32 with the reference's secant
+  fp64       plain FP64 (the reference's arithmetic)
+Errors of the FP32 + learned form against FP64 are measured after 1 and after
+`--error-steps` steps.  Every single-kernel operator is timed with HIP events
+on the working stream during the timed steps; `roofline` is for the one with
+the largest average launch time.  `cpu_baseline` times the CPU oracle (FP64 C
+restatement, OpenMP) for one step of the same workload on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -30,6 +34,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s per RK3 step: FP32+learned vs plain FP64 at <=1e-5 rel err"
+
+# Algorithmic HBM bytes per interior cell of one launch, FP32 storage (4 B):
+#   thermo  reads s, qt; writes T, qv, ql, alpha, buoyancy, theta_rho, N^2,
+#           thetali and the w tendency                          11 x 4
+#   adv_x   reads u, v, w, s, qt; writes the s, qt, u, v tendencies; reads
+#           and writes the w tendency                            11 x 4
+#   adv_y   reads the five fields; reads and writes five tendencies  15 x 4
+#   adv_z   as adv_y                                             15 x 4
+#   rk3     per variable: value and tendency in, value out, and the stage
+#           copy (written in stage 0, read in stages 1, 2); five variables  20 x 4
+KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80}
+KERNEL_NAMES = {"thermo": "thermo_sa_f32_kernel<true>", "adv_x": "adv::adv_x<PolicyF32,5,5,2,true>",
+                "adv_y": "adv::adv_y<PolicyF32,5,5,2,true>", "adv_z": "adv::adv_z<PolicyF32,5,5,2,true>",
+                "rk3": "rk3_kernel<float,...>"}
 
 
 def log(msg):
@@ -55,18 +74,26 @@ def build_state(cfg_name, nx=None, ny=None, nz=None):
     return c, nl, Gr, RS, Th, fields
 
 
-def run_form(nl, Gr, RS, Th, fields, precision, steps, warmup, learned=None):
-    import torch
+def make_sim(nl, Gr, RS, Th, fields, precision, learned=None):
     from pycles_amd.Simulation3d import Simulation3d
     nl = json.loads(json.dumps(nl))
     nl["mi355x"]["precision"] = precision
     sim = Simulation3d().initialize(nl, Gr=Gr, Ref=RS, Th=Th, fields=fields)
-    if learned is not None:
-        sim.Th.mlp = learned
+    sim.Th.mlp = learned
+    return sim
+
+
+def run_form(nl, Gr, RS, Th, fields, precision, steps, warmup, learned=None):
+    """Warm up, then time exactly `steps` steps between two synchronisations."""
+    import torch
+    sim = make_sim(nl, Gr, RS, Th, fields, precision, learned)
     sim.begin_run()
     for _ in range(warmup):
         sim.step()
     torch.cuda.synchronize()
+    counters = None
+    if hasattr(sim.Th, "counters"):
+        sim.Th.counters.zero_()
     sim.timers = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -75,67 +102,122 @@ def run_form(nl, Gr, RS, Th, fields, precision, steps, warmup, learned=None):
     t1 = time.perf_counter()
     phases = sim.phase_times()
     sim.timers = None
-    return sim, (t1 - t0) * 1e3 / steps, phases
+    if hasattr(sim.Th, "counters"):
+        counters = sim.Th.counters.cpu().numpy().tolist()
+    sim.Th.mlp = None
+    return sim, (t1 - t0) * 1e3 / steps, phases, counters
 
 
 def field_errors(sim32, sim64):
-    import torch
     out = {}
     pairs = [(n, "PV") for n in ("u", "v", "w", "s", "qt")] + [(n, "DV") for n in ("temperature", "ql")]
+    gw = sim64.Gr.dims.gw
     for n, kind in pairs:
-        if kind == "PV":
-            a = sim32.PV.get_device_layout(n)
-            b = sim64.PV.get_device_layout(n)
-        else:
-            a = sim32.DV.get_device_layout(n)
-            b = sim64.DV.get_device_layout(n)
-        gw = sim64.Gr.dims.gw
-        a, b = a[gw:-gw], b[gw:-gw]
-        scale = np.max(np.abs(b))
+        src32 = sim32.PV if kind == "PV" else sim32.DV
+        src64 = sim64.PV if kind == "PV" else sim64.DV
+        a = src32.get_device_layout(n)[gw:-gw]
+        b = src64.get_device_layout(n)[gw:-gw]
         d = np.abs(a - b)
         if n in ("temperature", "qt", "s"):
             rel = d / np.abs(b)
             out[n] = {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel * rel)))}
         else:
+            scale = float(np.max(np.abs(b)))
             out[n] = {"max_abs": float(d.max()), "rms_abs": float(np.sqrt(np.mean(d * d))),
                       "max_rel_to_max": float(d.max() / scale) if scale else 0.0}
-    del torch
     return out
 
 
-def cpu_baseline():
-    """FP64 CPU oracle, one SSP-RK3 step of the moist configuration on a
-    128x128x64 sample of the same spacing and profiles, on the host cores."""
-    from oracle import oracle as O
+def error_study(nl, Gr, RS, Th, fields, learned, nsteps):
+    """FP32 + learned against FP64 from the same state after 1 and nsteps steps."""
+    import torch
+    a = make_sim(nl, Gr, RS, Th, fields, "f32", learned)
+    b = make_sim(nl, Gr, RS, Th, fields, "f64")
+    out = {}
+    for sim in (a, b):
+        sim.Th.mlp = learned if sim is a else None
+        sim.begin_run()
+    for n in range(1, nsteps + 1):
+        for sim in (a, b):
+            sim.Th.mlp = learned if sim is a else None
+            sim.step()
+        if n == 1 or n == nsteps:
+            torch.cuda.synchronize()
+            out[f"after_{n}_steps"] = field_errors(a, b)
+            log(f"[bench] errors after {n} steps: T {out[f'after_{n}_steps']['temperature']}")
+    a.Th.mlp = None
+    return out
+
+
+def train_guess(Th, pairs_log2=22, epochs=40):
+    import torch
+    from pycles_amd.LearnedGuess import SaturationGuess
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = SaturationGuess(Th.CC, n_pairs=1 << pairs_log2, epochs=epochs).fit()
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    held = g.pairs(1 << 16, 424242)
+    T = held[:, 3].cpu().numpy()
+    y = g.predict(held).cpu().numpy().astype(np.float64)
+    err = y - T
+    info = {"pairs": g.n, "epochs": g.epochs, "batch": g.batch, "train_s": round(t_train, 3),
+            "loss_first_epoch": g.loss_history[0], "loss_last_epoch": g.loss_history[-1],
+            "heldout": "65536 saturated pairs, packed bf16 inference",
+            "heldout_rms_K": float(np.sqrt(np.mean(err ** 2))), "heldout_max_K": float(np.abs(err).max()),
+            "inference": "bf16 MFMA 32x32x16, FP32 accumulate, fused into the thermodynamics kernel"}
+    return g, info
+
+
+def per_kernel(phases, steps):
+    out = {}
+    for k, nbytes in KERNEL_BYTES.items():
+        if k not in phases:
+            continue
+        ms = phases[k] / (3 * steps)
+        out[k] = {"ms_per_launch": round(ms, 4), "bytes_per_cell": nbytes}
+    return out
+
+
+def cpu_baseline(c, Gr, RS, Th, fields):
+    """FP64 CPU oracle: one SSP-RK3 step of the same workload on the host cores."""
     from tests.helpers import oracle_sim
-    threads = min(os.cpu_count() or 1, 16)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    c, nl, Gr, RS, Th, fields = build_state("moist512", 128, 128, 64)
     sim = oracle_sim(Gr, RS, Th, fields, True, dt=c["dt"])
     sim.begin_run()
     t0 = time.perf_counter()
     sim.step()
     dt = time.perf_counter() - t0
-    n = 128 * 128 * 64
-    return {"value": round(n / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port",
-            "sample": "one SSP-RK3 step of the moist configuration on a 128x128x64 subdomain (1.05 M cells), "
-                      f"FP64 CPU oracle with OpenMP, {dt:.2f} s"}
+    n = c["nx"] * c["ny"] * c["nz"]
+    return {"value": round(n / dt / 1e6, 3), "unit": "Mcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"one SSP-RK3 step of the full {c['nx']}x{c['ny']}x{c['nz']} workload, FP64 CPU oracle "
+                      f"(C, OpenMP, {threads} threads): {dt:.2f} s"}
 
 
-# Algorithmic HBM bytes per cell for each operator in one stage (field passes
-# x storage bytes); see DESIGN.md "roofline accounting".
-PASSES = {"thermo": 2 + 1 + 1 + 8, "advection": 3 * 15, "rk3": 5 * 4, "pressure": 22}
+def measured_traffic():
+    """HBM bytes per launch of each kernel from committed rocprofv3 --pmc runs."""
+    p = os.path.join(ROOT, "profiles", "traffic_moist512.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return {}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="moist512")
+    ap.add_argument("--error-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--guess-pairs-log2", type=int, default=22)
+    ap.add_argument("--guess-epochs", type=int, default=40)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
@@ -146,43 +228,80 @@ def main():
     log(f"[bench] building {args.config} state")
     c, nl, Gr, RS, Th, fields = build_state(args.config)
     ncell = c["nx"] * c["ny"] * c["nz"]
-    log("[bench] FP32 form")
-    sim32, ms32, ph32 = run_form(nl, Gr, RS, Th, fields, "f32", args.steps, args.warmup)
-    log(f"[bench] FP32 {ms32:.3f} ms/step {ph32}")
-    res64 = None
+    mcell = lambda ms: ncell / (ms * 1e-3) / 1e6  # noqa: E731
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 1 GiB read + 1 GiB written")
+        del a, b
+    log("[bench] training the saturation guess")
+    guess, ginfo = train_guess(Th, args.guess_pairs_log2, args.guess_epochs)
+    log(f"[bench] trained in {ginfo['train_s']} s, held-out rms {ginfo['heldout_rms_K']:.4f} K")
+
+    log("[bench] FP32 + learned form")
+    simL, msL, phL, cntL = run_form(nl, Gr, RS, Th, fields, "f32", args.steps, args.warmup, learned=guess)
+    log(f"[bench] FP32+learned {msL:.3f} ms/step {phL}")
+    del simL
+    out_plain = None
+    if not args.no_plain:
+        simP, msP, phP, cntP = run_form(nl, Gr, RS, Th, fields, "f32", args.steps, args.warmup)
+        log(f"[bench] FP32 plain {msP:.3f} ms/step")
+        del simP
+        out_plain = {"ms_per_step": round(msP, 4), "value": round(mcell(msP), 2),
+                     "phases_ms_per_step": {k: round(v / args.steps, 4) for k, v in phP.items()},
+                     "sa_iterations_per_saturated_cell": round(cntP[1] / max(cntP[0], 1), 3)}
+    out64 = None
     if not args.no_fp64:
-        log("[bench] FP64 form")
-        sim64, ms64, ph64 = run_form(nl, Gr, RS, Th, fields, "f64", args.steps, args.warmup)
-        log(f"[bench] FP64 {ms64:.3f} ms/step {ph64}")
-        errs = field_errors(sim32, sim64)
-        res64 = {"ms_per_step": round(ms64, 4), "value": round(ncell / (ms64 * 1e-3) / 1e6, 2),
+        sim64, ms64, ph64, _ = run_form(nl, Gr, RS, Th, fields, "f64", args.steps, args.warmup)
+        log(f"[bench] FP64 {ms64:.3f} ms/step")
+        del sim64
+        out64 = {"ms_per_step": round(ms64, 4), "value": round(mcell(ms64), 2),
                  "phases_ms_per_step": {k: round(v / args.steps, 4) for k, v in ph64.items()}}
-    value = ncell / (ms32 * 1e-3) / 1e6
-    # dominant operator of the FP32 form
-    dom = max(ph32, key=ph32.get)
-    per_launch_ms = ph32[dom] / (3 * args.steps)
-    bytes_per_launch = PASSES[dom] * ncell * 4
-    achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
-    roof = {"bound": "hbm", "operator": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes_per_launch": bytes_per_launch, "ms_per_launch": round(per_launch_ms, 4)}
-    out = {"metric": "Mcell-updates/s per RK3 step: FP32+learned vs plain FP64 at <=1e-5 rel err",
-           "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms32, 4), "higher_is_better": True,
+    torch.cuda.empty_cache()
+
+    kernels = per_kernel(phL, args.steps)
+    traffic = measured_traffic()
+    for k, v in kernels.items():
+        ach = v["bytes_per_cell"] * ncell / (v["ms_per_launch"] * 1e-3) / 1e9
+        v["achieved_GBs"] = round(ach, 1)
+        v["hbm_frac"] = round(ach / HBM_PEAK_GBS, 4)
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_launch"])
+    kd = kernels[dom]
+    tr = traffic.get(dom)
+    roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": kd["hbm_frac"],
+            "traffic": tr["bytes_per_launch"] if tr else None,
+            "algorithmic_bytes_per_launch": kd["bytes_per_cell"] * ncell, "ms_per_launch": kd["ms_per_launch"]}
+    ginfo["newton_steps"] = Th.newton_steps
+    ginfo["saturated_fraction"] = round(cntL[0] / (ncell * 3 * args.steps), 4)
+    ginfo["newton_iterations_per_saturated_cell"] = round(cntL[1] / max(cntL[0], 1), 3)
+    ginfo["fallback_fraction"] = cntL[3] / max(cntL[0], 1)
+
+    out = {"metric": METRIC, "value": round(mcell(msL), 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(msL, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-           "data": "synthetic (seeded BOMEX-like moist state)",
+           "data": "synthetic (seeded BOMEX-like moist state with a saturated layer)",
            "config": {"workload": f"{args.config}: {c['nx']}x{c['ny']}x{c['nz']} moist SA, WENO{c['order'][0]}/"
-                                  f"{c['order'][1]}, FFT+tridiagonal pressure, dt={c['dt']} s",
-                      "cells": ncell},
-           "phases_ms_per_step": {k: round(v / args.steps, 4) for k, v in ph32.items()},
-           "roofline": roof}
-    if res64:
-        out["fp64"] = res64
-        out["speedup_vs_fp64"] = round(res64["ms_per_step"] / ms32, 3)
-        out["fp32_vs_fp64"] = errs
+                                  f"{c['order'][1]}, FFT+tridiagonal pressure, dt={c['dt']} s, "
+                                  "FP32 + learned saturation guess", "cells": ncell},
+           "phases_ms_per_step": {k: round(v / args.steps, 4) for k, v in phL.items()},
+           "roofline": roof, "kernels": kernels, "learned_guess": ginfo}
+    if out64:
+        out["fp64"] = out64
+        out["speedup_vs_fp64"] = round(out64["ms_per_step"] / msL, 3)
+    if out_plain:
+        out["fp32_plain"] = out_plain
+    if not args.no_fp64 and args.error_steps > 0:
+        log(f"[bench] error study: {args.error_steps} steps")
+        out["errors_vs_fp64"] = error_study(nl, Gr, RS, Th, fields, guess, args.error_steps)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         log("[bench] CPU baseline")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(c, Gr, RS, Th, fields)
     print(json.dumps(out), flush=True)
 
 

@@ -27,6 +27,7 @@
 #ifndef PYCLES_AMD_H
 #define PYCLES_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -87,8 +88,13 @@ int pycles_device_count(void);
  * least (order+1)/2.
  * flags: PYCLES_ADV_TEND_ZERO declares that the scalar, u and v tendencies
  * are zero on entry (the w tendency is still read: it carries buoyancy); they
- * are then written instead of read-modified-written, with identical bits. */
+ * are then written instead of read-modified-written, with identical bits.
+ * PYCLES_ADV_PASS_X/Y/Z: when any is set, only the selected directional
+ * passes run (for timing one kernel; a partial call is not a full update). */
 #define PYCLES_ADV_TEND_ZERO 1
+#define PYCLES_ADV_PASS_X 2
+#define PYCLES_ADV_PASS_Y 4
+#define PYCLES_ADV_PASS_Z 8
 int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
                      void *const tend_vel[3], int nscal, const void *const *scal,
                      void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
@@ -103,14 +109,16 @@ typedef struct pycles_lookup {
 } pycles_lookup;
 
 /* Saturation-adjustment counters (device int64[4]): saturated cells,
- * secant iterations, cells that hit the iteration cap, cells that fell back
- * from the learned guess to the secant.  May be NULL. */
+ * iterations (Newton corrections after the learned guess plus secant steps),
+ * cells that hit the secant cap, and cells whose first FP32 result failed the
+ * FP64 residual check (a failed guess is redone by the FP32 secant, a failed
+ * FP32 secant by the FP64 secant).  May be NULL. */
 typedef struct pycles_sa_options {
     int do_qt_clipping;  /* ThermodynamicsSA.pyx:60-67 */
     int max_iter;        /* secant cap (the reference loop is uncapped) */
     int latent_heat;     /* PYCLES_LH_* */
-    const void *mlp;     /* learned first guess (pycles_mlp handle) or NULL */
-    int newton_steps;    /* Newton corrections after the learned guess */
+    const void *mlp;     /* packed learned first guess (pycles_mlp_pack) or NULL; PYCLES_F32 only */
+    int newton_steps;    /* Newton corrections after the learned guess, 1..4 */
     long long *counters; /* device int64[4] or NULL */
     int wt_overwrite;    /* 1: the w tendency is zero on entry, write instead of add */
 } pycles_sa_options;
@@ -184,6 +192,52 @@ int pycles_layout_ingress(const pycles_dims *dims, const double *ref_layout, voi
  * plus base (may be NULL). */
 int pycles_layout_egress(const pycles_dims *dims, const void *dev, double *ref_layout,
                          const double *base, int precision, void *stream);
+
+/* ---- learned first guess of the saturation adjustment -----------------
+ * No counterpart in the reference: its eos_c (thermodynamics_sa.h:17-66)
+ * starts the secant from the unsaturated temperature.  The network maps
+ * normalised (s, qt, p0) to T with layers 3 -> 64 -> 64 -> 1 (tanh).
+ * Training state lives in caller-owned device buffers: params and the two
+ * Adam moments (pycles_mlp_param_count() floats each), the normalisation
+ * (pycles_mlp_norm_bytes()), a workspace (pycles_mlp_workspace_bytes(batch))
+ * and the packed inference block (pycles_mlp_weights_bytes(), 16-byte
+ * aligned) that pycles_sa_options.mlp points to.  Training data is a device
+ * array [n][4] of (s, qt, p0, T) in FP64. */
+
+/* n training pairs from the FP64 saturation adjustment (eos_c) on
+ * qt ~ U(0, 0.025), p0 ~ U(5e4, 1.02e5) Pa, T ~ U(230, 310) K; with
+ * saturated_only, the same box conditioned on qt > qv*(T) (the guess is
+ * only used on the saturated branch). */
+int pycles_sa_training_pairs(const pycles_lookup *lt, long long n, unsigned long long seed, int max_iter,
+                             int saturated_only, double *out, void *stream);
+size_t pycles_mlp_param_count(void);
+size_t pycles_mlp_norm_bytes(void);
+size_t pycles_mlp_weights_bytes(void);
+size_t pycles_mlp_workspace_bytes(int batch);
+/* Glorot-uniform weights, zero biases; zeroes the Adam moments when given. */
+int pycles_mlp_init(float *params, float *adam_m, float *adam_v, unsigned long long seed, void *stream);
+/* Mean and standard deviation of the four columns of data. */
+int pycles_mlp_normalisation(const double *data, long long n, void *norm, void *workspace, void *stream);
+/* Gradient of the mean of 0.5 (y - t)^2 (normalised T) over the first n
+ * samples (n a multiple of 128); *loss receives the sum of 0.5 (y - t)^2. */
+int pycles_mlp_grad(const float *params, const void *norm, const double *data, long long n, float *grad,
+                    double *loss, void *workspace, void *stream);
+/* `epochs` passes of Adam over shuffled minibatches (n a power of two, batch a
+ * multiple of 128); cosine learning rate from lr_max to lr_min over
+ * total_steps (<= 0: this call's steps); step0 = Adam steps already taken.
+ * epoch_loss (device, epochs doubles, may be NULL) accumulates the summed
+ * 0.5 (y - t)^2 of each epoch. */
+int pycles_mlp_train(float *params, float *adam_m, float *adam_v, const void *norm, const double *data,
+                     long long n, int batch, int epochs, float lr_max, float lr_min, unsigned long long seed,
+                     long long step0, long long total_steps, double *epoch_loss, void *workspace, void *stream);
+/* FP32 forward pass with the FP32 weights: out[i] = T guess (K). */
+int pycles_mlp_forward(const float *params, const void *norm, const double *data, long long n, float *out,
+                       void *stream);
+/* Pack params (weights rounded to bf16) and normalisation for inference. */
+int pycles_mlp_pack(const float *params, const void *norm, void *weights, void *stream);
+/* The thermodynamics kernel's inference path (bf16 MFMA, FP32 accumulate)
+ * applied to the (s, qt, p0) columns of data. */
+int pycles_mlp_infer(const void *weights, const double *data, long long n, float *out, void *stream);
 
 #ifdef __cplusplus
 }

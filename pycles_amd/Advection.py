@@ -75,13 +75,15 @@ class MomentumAdvection:
                         PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
 
 
-def advect_fused(SA: ScalarAdvection, MA: MomentumAdvection, Gr, RS, PV, tend_zero=False):
+def advect_fused(SA: ScalarAdvection, MA: MomentumAdvection, Gr, RS, PV, tend_zero=False, passes=0):
     """SA.update followed by MA.update in one launch sequence.  tend_zero:
     the scalar, u and v tendencies are zero (or stale and to be replaced) on
-    entry, so they are written rather than accumulated (same bits)."""
+    entry, so they are written rather than accumulated (same bits).  passes:
+    0 for the full update, or an OR of _capi.ADV_PASS_X/Y/Z to run only those
+    directional kernels (timing)."""
     names = _scalar_names(PV)
     ref = D.device_ref(RS)
     tv = [PV.tptr(PV.velocity_names_directional[d]) for d in range(3)]
     _capi.advection(SA.dims, ref.c, _vel(PV), tv, [PV.ptr(n) for n in names],
                     [PV.tptr(n) for n in names], _base(PV, names), SA.order, MA.order,
-                    PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
+                    PV.precision, D.stream(), (_capi.ADV_TEND_ZERO if tend_zero else 0) | passes)

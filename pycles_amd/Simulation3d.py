@@ -107,10 +107,19 @@ class Simulation3d:
         self._mark("thermo")
         self.Th.update(self.Gr, self.Ref, self.PV, self.DV, wt_overwrite=fast and not self._wt_pending)
         self._wt_pending = False
-        self._mark("advection")
-        if self.fuse_advection:
+        if self.fuse_advection and self.timers is not None:
+            # the same three kernels as the fused call, with an event between
+            # each so every directional pass is timed on its own
+            from ._device import _capi
+            for name, bit in (("adv_x", _capi.ADV_PASS_X), ("adv_y", _capi.ADV_PASS_Y),
+                              ("adv_z", _capi.ADV_PASS_Z)):
+                self._mark(name)
+                advect_fused(self.SA, self.MA, self.Gr, self.Ref, self.PV, tend_zero=fast, passes=bit)
+        elif self.fuse_advection:
+            self._mark("advection")
             advect_fused(self.SA, self.MA, self.Gr, self.Ref, self.PV, tend_zero=fast)
         else:
+            self._mark("advection")
             self.SA.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa, tend_zero=fast)
             self.MA.update(self.Gr, self.Ref, self.PV, self.Pa, tend_zero=fast)
         self._mark("rk3")

@@ -48,6 +48,9 @@ cdef extern from "pycles_amd.h":
     int PYCLES_LH_VARIABLE
     int PYCLES_LH_CONSTANT
     int PYCLES_ADV_TEND_ZERO
+    int PYCLES_ADV_PASS_X
+    int PYCLES_ADV_PASS_Y
+    int PYCLES_ADV_PASS_Z
 
     const char *pycles_last_error()
     const char *pycles_version()
@@ -85,12 +88,33 @@ cdef extern from "pycles_amd.h":
                               const double *base, int precision, void *stream)
     int pycles_layout_egress(const pycles_dims *dims, const void *dev, double *ref_layout,
                              const double *base, int precision, void *stream)
+    int pycles_sa_training_pairs(const pycles_lookup *lt, long long n, unsigned long long seed,
+                                 int max_iter, int saturated_only, double *out, void *stream)
+    size_t pycles_mlp_param_count()
+    size_t pycles_mlp_norm_bytes()
+    size_t pycles_mlp_weights_bytes()
+    size_t pycles_mlp_workspace_bytes(int batch)
+    int pycles_mlp_init(float *params, float *adam_m, float *adam_v, unsigned long long seed, void *stream)
+    int pycles_mlp_normalisation(const double *data, long long n, void *norm, void *workspace, void *stream)
+    int pycles_mlp_grad(const float *params, const void *norm, const double *data, long long n, float *grad,
+                        double *loss, void *workspace, void *stream)
+    int pycles_mlp_train(float *params, float *adam_m, float *adam_v, const void *norm, const double *data,
+                         long long n, int batch, int epochs, float lr_max, float lr_min,
+                         unsigned long long seed, long long step0, long long total_steps,
+                         double *epoch_loss, void *workspace, void *stream)
+    int pycles_mlp_forward(const float *params, const void *norm, const double *data, long long n,
+                           float *out, void *stream)
+    int pycles_mlp_pack(const float *params, const void *norm, void *weights, void *stream)
+    int pycles_mlp_infer(const void *weights, const double *data, long long n, float *out, void *stream)
 
 F64 = PYCLES_F64
 F32 = PYCLES_F32
 LH_VARIABLE = PYCLES_LH_VARIABLE
 LH_CONSTANT = PYCLES_LH_CONSTANT
 ADV_TEND_ZERO = PYCLES_ADV_TEND_ZERO
+ADV_PASS_X = PYCLES_ADV_PASS_X
+ADV_PASS_Y = PYCLES_ADV_PASS_Y
+ADV_PASS_Z = PYCLES_ADV_PASS_Z
 
 
 class PyclesError(RuntimeError):
@@ -274,3 +298,60 @@ def layout_ingress(Dims dims, ref_layout, dev, base, int precision, stream):
 def layout_egress(Dims dims, dev, ref_layout, base, int precision, stream):
     _check(pycles_layout_egress(&dims.d, _p(dev), <double *> _p(ref_layout),
                                 <const double *> _p(base), precision, _p(stream)))
+
+
+# ---- learned first guess of the saturation adjustment --------------------
+
+def sa_training_pairs(Lookup lt, long long n, unsigned long long seed, int max_iter, out, stream,
+                      int saturated_only=1):
+    _check(pycles_sa_training_pairs(&lt.t, n, seed, max_iter, saturated_only, <double *> _p(out), _p(stream)))
+
+
+def mlp_param_count():
+    return pycles_mlp_param_count()
+
+
+def mlp_norm_bytes():
+    return pycles_mlp_norm_bytes()
+
+
+def mlp_weights_bytes():
+    return pycles_mlp_weights_bytes()
+
+
+def mlp_workspace_bytes(int batch):
+    return pycles_mlp_workspace_bytes(batch)
+
+
+def mlp_init(params, adam_m, adam_v, unsigned long long seed, stream):
+    _check(pycles_mlp_init(<float *> _p(params), <float *> _p(adam_m), <float *> _p(adam_v), seed, _p(stream)))
+
+
+def mlp_normalisation(data, long long n, norm, workspace, stream):
+    _check(pycles_mlp_normalisation(<const double *> _p(data), n, _p(norm), _p(workspace), _p(stream)))
+
+
+def mlp_grad(params, norm, data, long long n, grad, loss, workspace, stream):
+    _check(pycles_mlp_grad(<const float *> _p(params), _p(norm), <const double *> _p(data), n,
+                           <float *> _p(grad), <double *> _p(loss), _p(workspace), _p(stream)))
+
+
+def mlp_train(params, adam_m, adam_v, norm, data, long long n, int batch, int epochs, float lr_max,
+              float lr_min, unsigned long long seed, long long step0, long long total_steps, epoch_loss,
+              workspace, stream):
+    _check(pycles_mlp_train(<float *> _p(params), <float *> _p(adam_m), <float *> _p(adam_v), _p(norm),
+                            <const double *> _p(data), n, batch, epochs, lr_max, lr_min, seed, step0,
+                            total_steps, <double *> _p(epoch_loss), _p(workspace), _p(stream)))
+
+
+def mlp_forward(params, norm, data, long long n, out, stream):
+    _check(pycles_mlp_forward(<const float *> _p(params), _p(norm), <const double *> _p(data), n,
+                              <float *> _p(out), _p(stream)))
+
+
+def mlp_pack(params, norm, weights, stream):
+    _check(pycles_mlp_pack(<const float *> _p(params), _p(norm), _p(weights), _p(stream)))
+
+
+def mlp_infer(weights, data, long long n, out, stream):
+    _check(pycles_mlp_infer(_p(weights), <const double *> _p(data), n, <float *> _p(out), _p(stream)))

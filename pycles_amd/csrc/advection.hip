@@ -53,6 +53,8 @@ extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, 
     adv::Fields fl;
     memset(&fl, 0, sizeof(fl));
     fl.tzero = (flags & PYCLES_ADV_TEND_ZERO) ? 1 : 0;
+    fl.passes = (flags >> 1) & 7;
+    if (!fl.passes) fl.passes = 7;
     for (int s = 0; s < nscal; ++s) {
         fl.q[s] = scal[s];
         fl.t[s] = tend_scal[s];

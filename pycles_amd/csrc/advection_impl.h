@@ -33,6 +33,7 @@ struct Fields {
     void *t[5];        // tendencies
     const double *base[2];
     int tzero;  // scalar, u, v tendencies are zero on entry (x pass writes them)
+    int passes;  // bit 0: x, bit 1: y, bit 2: z
 };
 
 struct Prof {
@@ -538,23 +539,29 @@ int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
         }
     }
     int bx = g.nx >= 256 ? 256 : ((g.nx + 63) / 64) * 64;
-    hipLaunchKernelGGL((adv_x<P, SO, MO, NSC, MOM>), dim3(g.ny, g.nz), dim3(bx), smem, st, g, pr, fl);
-    PYCLES_CHECK_LAUNCH();
+    if (fl.passes & 1) {
+        hipLaunchKernelGGL((adv_x<P, SO, MO, NSC, MOM>), dim3(g.ny, g.nz), dim3(bx), smem, st, g, pr, fl);
+        PYCLES_CHECK_LAUNCH();
+    }
     // y: split the march into chunks so the grid holds enough waves.
     const long long ycols = (long long)((g.nx + 63) / 64) * ((g.nz + 3) / 4);
     int ych = 1;
     while (ycols * ych < 2048 && g.ny / (ych * 2) >= 32) ych *= 2;
     const int jchunk = (g.ny + ych - 1) / ych;
-    hipLaunchKernelGGL((adv_y<P, SO, MO, NSC, MOM>), dim3((g.nx + 63) / 64, (g.nz + 3) / 4, ych), dim3(64, 4),
-                       0, st, g, pr, fl, jchunk);
-    PYCLES_CHECK_LAUNCH();
+    if (fl.passes & 2) {
+        hipLaunchKernelGGL((adv_y<P, SO, MO, NSC, MOM>), dim3((g.nx + 63) / 64, (g.nz + 3) / 4, ych),
+                           dim3(64, 4), 0, st, g, pr, fl, jchunk);
+        PYCLES_CHECK_LAUNCH();
+    }
     const long long zcols = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
     int zch = 1;
     while (zcols * zch < 2048 && g.nz / (zch * 2) >= 32) zch *= 2;
     const int kchunk = (g.nz + zch - 1) / zch;
-    hipLaunchKernelGGL((adv_z<P, SO, MO, NSC, MOM>), dim3((g.nx + 63) / 64, (g.ny + 3) / 4, zch), dim3(64, 4),
-                       0, st, g, pr, fl, kchunk);
-    PYCLES_CHECK_LAUNCH();
+    if (fl.passes & 4) {
+        hipLaunchKernelGGL((adv_z<P, SO, MO, NSC, MOM>), dim3((g.nx + 63) / 64, (g.ny + 3) / 4, zch),
+                           dim3(64, 4), 0, st, g, pr, fl, kchunk);
+        PYCLES_CHECK_LAUNCH();
+    }
     return PYCLES_OK;
 }
 

@@ -1,0 +1,168 @@
+// FP32 saturation-adjustment kernel (with and without the learned first
+// guess).  Built with multiply-add contraction: the FP32 form is checked
+// against the FP64 form by tolerance, not bit for bit.  See thermo.hip for the
+// reference citations.
+#include "thermo_common.h"
+
+// ---------------------------------------------------------------------------
+// FP32 storage and arithmetic; s and qt are deviations from FP64 level
+// profiles; FP64 only in the per-level constants and the residual check.
+// MLP selects the learned first guess (a separate instantiation, so the plain
+// kernel carries none of the network's registers or LDS).  The guess is
+// evaluated by whole wavefronts (one row of 64 columns at one level) when any
+// of their cells is saturated; every lane stays in the loop for that reason.
+// Dynamic LDS: the pv* table in FP64 and FP32, then the chunk's exner values.
+struct NoSmem {};
+struct MlpSmem {
+    mlp::Weights w;
+    alignas(16) float red[4][mlp::RED_FLOATS];
+};
+
+template <bool MLP>
+__global__ __launch_bounds__(256, 4) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
+    extern __shared__ double dyn[];
+    double *tab = dyn;
+    float *tabf = (float *)(tab + a.ntab);
+    float *exner = tabf + a.ntab;
+    __shared__ unsigned long long cnt[4];
+    __shared__ typename std::conditional<MLP, MlpSmem, NoSmem>::type msm_store;
+    MlpSmem *msm = reinterpret_cast<MlpSmem *>(&msm_store);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nthr = blockDim.x * blockDim.y;
+    const Chunk ch = chunk_of(g, a.kchunk);
+    for (int t = tid; t < a.ntab; t += nthr) {
+        tab[t] = a.tab[t];
+        tabf[t] = (float)a.tab[t];
+    }
+    for (int k = ch.kw + tid; k < ch.ke; k += nthr) exner[k - ch.kw] = (float)thermo::exner_c(a.p0[k]);
+    if (tid < 4) cnt[tid] = 0;
+    if (MLP) mlp::stage(a.mlp, &msm->w, tid, nthr);
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    unsigned long long ctr[4] = {0, 0, 0, 0};
+    const bool active = i < g.nx && j < g.ny;
+    const long long col = active ? (long long)j * g.nx + i : 0;
+    float *s = (float *)a.s;
+    float *qtp = (float *)a.qt;
+    float *wt = (float *)a.wt;
+    float *To = (float *)a.T, *qvo = (float *)a.qv, *qlo = (float *)a.ql, *alo = (float *)a.alpha;
+    float *bo = (float *)a.buoy, *thro = (float *)a.thr, *bvfo = (float *)a.bvf, *thlo = (float *)a.thl;
+    const thermo32::TableF ltf{tabf, (float)a.x_min, (float)a.dx, (float)a.dxi, a.ntab};
+    const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
+    const float dzi = (float)(1.0 / g.dx[2]);
+    float Tw[3] = {0, 0, 0}, qtw[3] = {0, 0, 0}, qvw[3] = {0, 0, 0}, thw[3] = {0, 0, 0}, bprev = 0.0f;
+    for (int k = ch.kw; k < ch.ke; ++k) {
+        const bool own = active && k >= ch.k0 && k < ch.k1;
+        const bool own_m = active && k - 1 >= ch.k0 && k - 1 < ch.k1;
+        const long long o = (long long)k * g.plane + col;
+        const double p0 = a.p0[k];
+        const float p0f = (float)p0;
+        const double sb = a.sbase ? a.sbase[k] : 0.0;
+        const double qb = a.qbase ? a.qbase[k] : 0.0;
+        float sd = active ? s[o] : 0.0f;
+        float qd = active ? qtp[o] : 0.0f;
+        double qt64 = qb + (double)qd;
+        if (a.clip && !(qt64 >= 1e-11)) {
+            qt64 = 1e-11;
+            if (own) qtp[o] = (float)(1e-11 - qb);
+        }
+        const float qt = (float)qt64;
+        const double s64 = sb + (double)sd;
+        // s - (1-qt) sd~ - qt sv~ formed from the level profile and the deviations
+        const double s_red = (sb - PC_SD_TILDE - qb * (PC_SV_TILDE - PC_SD_TILDE)) + (double)sd -
+                             (qt64 - qb) * (PC_SV_TILDE - PC_SD_TILDE);
+        const thermo32::SAStart st = thermo32::eos_sa_f32_start(ltf, p0f, s_red, qt);
+        const bool sat = active && st.saturated;
+        float T_guess = -1.0f;
+        if (MLP) {
+            if (__ballot(sat)) {
+                float x0, x1, x2;
+                mlp::normalise(&msm->w, s64, qt, p0f, &x0, &x1, &x2);
+                T_guess = mlp::guess_wave(&msm->w, msm->red[threadIdx.y], x0, x1, x2);
+            }
+        }
+        float T = st.T_1, qv = qt, ql = 0.0f;
+        int it = 0, fb = 0;
+        if (sat)
+            it = thermo32::eos_sa_f32_finish(ltf, lt, st, p0f, p0, s64, s_red, qt, qt64, T_guess, a.newton, &T,
+                                             &qv, &ql, a.max_iter, &fb);
+        if (sat && own) {
+            ctr[0] += 1;
+            ctr[1] += it;
+            if (T_guess <= 0.0f && a.max_iter > 0 && it >= a.max_iter) ctr[2] += 1;
+            ctr[3] += fb;
+        }
+        const float al = (float)PC_RD * T / p0f * (1.0f - qt + (float)PC_EPS_VI * qv);
+        const float ex = exner[k - ch.kw];
+        float b = 0.0f;
+        if (k < g.nzg - 1) {
+            const float a0 = (float)a.alpha0_half[k];
+            b = (float)PC_G * (al - a0) / a0;
+        }
+        float thr = 0.0f;
+        if (k >= 1 && k < g.nzg - 1) thr = T * (1.0f - qt + (float)PC_EPS_VI * qv) / ex;
+        Tw[0] = Tw[1]; Tw[1] = Tw[2]; Tw[2] = T;
+        qtw[0] = qtw[1]; qtw[1] = qtw[2]; qtw[2] = qt;
+        qvw[0] = qvw[1]; qvw[1] = qvw[2]; qvw[2] = qv;
+        thw[0] = thw[1]; thw[1] = thw[2]; thw[2] = thr;
+        if (own) {
+            To[o] = T;
+            qvo[o] = qv;
+            qlo[o] = ql;
+            alo[o] = al;
+            if (bo && k < g.nzg - 1) bo[o] = b;
+            if (thro && k >= 1 && k < g.nzg - 1) thro[o] = thr;
+            if (thlo) {
+                if (ql == 0.0f) thlo[o] = T / ex;
+                else {
+                    const float L = thermo32::latent_heat(T);
+                    thlo[o] = T / ex * expf(-L * (ql / (1.0f - qt)) / (T * (float)PC_CPD));
+                }
+            }
+        }
+        if (own_m && k >= 2 && k - 1 < g.nzg - 3) {
+            const float w0 = a.wt_over ? 0.0f : wt[o - g.plane];
+            wt[o - g.plane] = w0 + 0.5f * (bprev + b);
+        }
+        bprev = b;
+        const int m = k - 1;
+        if (bvfo && own_m && m >= 2 && m < g.nzg - 2) {
+            float bv;
+            if (qvw[1] < qtw[1]) {
+                const float Tk = Tw[1];
+                const float Lv = thermo32::latent_heat(Tk);
+                const float pvs = thermo32::lookup(ltf, Tk);
+                const float p0m = (float)a.p0[m];
+                const float rsl = (float)PC_EPS_V * pvs / (p0m - pvs);
+                const float cpm = (1.0f - qtw[1]) * (float)PC_CPD + qtw[1] * (float)PC_CPV;
+                const float gamma_w = (float)(PC_G / PC_CPD) * (1.0f / (1.0f - qtw[1])) *
+                                      (1.0f + Lv * rsl / ((float)PC_RD * Tk)) /
+                                      (cpm / (float)PC_CPD + Lv * Lv * ((float)PC_EPS_V + rsl) * rsl /
+                                                                 ((float)(PC_CPD * PC_RD) * Tk * Tk));
+                const float dTdz = 0.5f * (Tw[2] - Tw[0]) * dzi;
+                const float dqtdz = 0.5f * (qtw[2] - qtw[0]) * dzi;
+                bv = (float)PC_G / Tk * (dTdz + gamma_w) * (1.0f + Lv * rsl / ((float)PC_RD * Tk)) -
+                     dqtdz / (1.0f - qtw[1]);
+            } else {
+                bv = (float)PC_G / thw[1] * 0.5f * (thw[2] - thw[0]) * dzi;
+            }
+            bvfo[o - g.plane] = bv;
+        }
+    }
+    flush_counters(cnt, ctr, a.counters, tid);
+}
+
+// Dynamic LDS of the FP32 kernel: table (FP64 + FP32) and the chunk's exner.
+static size_t f32_smem_bytes(const SAArgs &a) {
+    return (size_t)a.ntab * (sizeof(double) + sizeof(float)) + (size_t)(a.kchunk + 2) * sizeof(float);
+}
+
+int thermo_sa_f32_launch(const Geo &g, const SAArgs &a, dim3 grd, dim3 blk, hipStream_t st) {
+    if (a.mlp)
+        hipLaunchKernelGGL(thermo_sa_f32_kernel<true>, grd, blk, f32_smem_bytes(a), st, g, a);
+    else
+        hipLaunchKernelGGL(thermo_sa_f32_kernel<false>, grd, blk, f32_smem_bytes(a), st, g, a);
+    PYCLES_CHECK_LAUNCH();
+    return PYCLES_OK;
+}
