@@ -45,10 +45,14 @@ METRIC = "Mcell-updates/s per RK3 step: FP32+learned vs plain FP64 at <=1e-5 rel
 #   adv_z   as adv_y                                             15 x 4
 #   rk3     per variable: value and tendency in, value out, and the stage
 #           copy (written in stage 0, read in stages 1, 2); five variables  20 x 4
-KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80}
+#           (only when the RK3 update is not fused into the z pass)
+#   adv_z_rk3  the z pass with the RK3 stage fused: reads the five fields and
+#           their tendencies, reads (stages 1, 2) or writes (stage 0) the stage
+#           copies, writes the five new values                   20 x 4
+KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, "adv_z_rk3": 80}
 KERNEL_NAMES = {"thermo": "thermo_sa_f32_kernel<true>", "adv_x": "adv::adv_x<PolicyF32,5,5,2,true>",
                 "adv_y": "adv::adv_y<PolicyF32,5,5,2,true>", "adv_z": "adv::adv_z<PolicyF32,5,5,2,true>",
-                "rk3": "rk3_kernel<float,...>"}
+                "rk3": "rk3_kernel<float,...>", "adv_z_rk3": "adv::adv_z<PolicyF32,5,5,2,true> (RK3 fused)"}
 
 
 def log(msg):

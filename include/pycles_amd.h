@@ -99,6 +99,21 @@ int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void 
                      void *const tend_vel[3], int nscal, const void *const *scal,
                      void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
                      int momentum_order, int precision, int flags, void *stream);
+/* pycles_advection followed by the RK3 stage in the same sweep: TimeStepping.
+ * update_third (TimeStepping.pyx:131-151) and the z boundary fill of
+ * PrognosticVariables.update_all_bcs (PrognosticVariables.pyx:183-233;
+ * sym for the scalars, u and v, asym for w).  The z pass applies the stage to
+ * each cell as soon as its tendency is final and writes the new value, and
+ * the ghost levels it feeds, to out_* (arrays distinct from the inputs; the
+ * caller swaps them in).  copy_* are the stage copies (written at rk_step 0,
+ * read at 1 and 2).  The tendency arrays keep the x and y contributions.
+ * Same bits as pycles_advection then pycles_rk3_stage.  Needs the momentum. */
+int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                         void *const tend_vel[3], int nscal, const void *const *scal,
+                         void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
+                         int momentum_order, int precision, int flags, void *const out_vel[3],
+                         void *const *out_scal, void *const copy_vel[3], void *const *copy_scal,
+                         int rk_step, double dt, void *stream);
 
 /* ---- thermodynamics ---------------------------------------------------- */
 /* pv*(T) lookup table (reference Csrc/lookup.h:4-54). */

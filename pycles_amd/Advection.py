@@ -75,6 +75,21 @@ class MomentumAdvection:
                         PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
 
 
+def advect_rk3(SA: ScalarAdvection, MA: MomentumAdvection, TS, Gr, RS, PV, tend_zero=False, passes=0):
+    """advect_fused followed by TS.update in one sweep (pycles_advection_rk3):
+    the new values go to PV's second array, which is then made current."""
+    names = _scalar_names(PV)
+    ref = D.device_ref(RS)
+    vn = PV.velocity_names_directional
+    tv = [PV.tptr(vn[d]) for d in range(3)]
+    _capi.advection_rk3(SA.dims, ref.c, _vel(PV), tv, [PV.ptr(n) for n in names],
+                        [PV.tptr(n) for n in names], _base(PV, names), SA.order, MA.order,
+                        PV.precision, D.stream(), (_capi.ADV_TEND_ZERO if tend_zero else 0) | passes,
+                        [PV.ptr_next(vn[d]) for d in range(3)], [PV.ptr_next(n) for n in names],
+                        [TS.copy_ptr(PV, vn[d]) for d in range(3)], [TS.copy_ptr(PV, n) for n in names],
+                        TS.rk_step, TS.dt)
+
+
 def advect_fused(SA: ScalarAdvection, MA: MomentumAdvection, Gr, RS, PV, tend_zero=False, passes=0):
     """SA.update followed by MA.update in one launch sequence.  tend_zero:
     the scalar, u and v tendencies are zero (or stale and to be replaced) on

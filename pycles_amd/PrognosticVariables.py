@@ -87,6 +87,16 @@ class PrognosticVariables(_State):
         self.tendencies = torch.zeros(shape, dtype=dt, device=dev)
         self.Gr = Gr
 
+    def ptr_next(self, name) -> int:
+        """The second value array, written by the fused advection + RK3 stage."""
+        if getattr(self, "values_next", None) is None:
+            self.values_next = torch.zeros_like(self.values)
+        return self.values_next[self.name_index[name]].data_ptr()
+
+    def swap_values(self):
+        """Make the array the fused stage wrote the current one."""
+        self.values, self.values_next = self.values_next, self.values
+
     def tend(self, name) -> torch.Tensor:
         return self.tendencies[self.name_index[name]]
 

@@ -15,7 +15,7 @@ from __future__ import annotations
 import time
 
 from . import _device as D
-from .Advection import MomentumAdvection, ScalarAdvection, advect_fused
+from .Advection import MomentumAdvection, ScalarAdvection, advect_fused, advect_rk3
 from .Grid import Grid
 from .ParallelMPI import NetCDFIO_Stats, ParallelMPI
 from .PressureSolver import PressureSolver
@@ -31,6 +31,8 @@ class Simulation3d:
     def __init__(self, namelist=None):
         self.fuse_advection = True
         self.write_tendencies = True
+        # the RK3 update runs inside the advection's z pass (needs the two above)
+        self.fuse_rk = True
         self._wt_pending = False
 
     def initialize(self, namelist, Gr=None, Ref=None, Th=None, fields=None):
@@ -107,7 +109,20 @@ class Simulation3d:
         self._mark("thermo")
         self.Th.update(self.Gr, self.Ref, self.PV, self.DV, wt_overwrite=fast and not self._wt_pending)
         self._wt_pending = False
-        if self.fuse_advection and self.timers is not None:
+        fused_rk = self.fuse_rk and self.fuse_advection and fast
+        if fused_rk:
+            if self.timers is not None:
+                from ._device import _capi
+                passes = (("adv_x", _capi.ADV_PASS_X), ("adv_y", _capi.ADV_PASS_Y),
+                          ("adv_z_rk3", _capi.ADV_PASS_Z))
+            else:
+                passes = (("advection_rk3", 0),)
+            for name, bit in passes:
+                self._mark(name)
+                advect_rk3(self.SA, self.MA, self.TS, self.Gr, self.Ref, self.PV, tend_zero=fast, passes=bit)
+            self.PV.swap_values()
+            self.TS.advance()
+        elif self.fuse_advection and self.timers is not None:
             # the same three kernels as the fused call, with an event between
             # each so every directional pass is timed on its own
             from ._device import _capi
@@ -122,8 +137,9 @@ class Simulation3d:
             self._mark("advection")
             self.SA.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa, tend_zero=fast)
             self.MA.update(self.Gr, self.Ref, self.PV, self.Pa, tend_zero=fast)
-        self._mark("rk3")
-        self.TS.update(self.Gr, self.PV, self.Pa, zero_tendencies=not fast)
+        if not fused_rk:
+            self._mark("rk3")
+            self.TS.update(self.Gr, self.PV, self.Pa, zero_tendencies=not fast)
         last = rk_step == self.TS.n_rk_steps - 1
         self._mark("pressure")
         self.Pr.update(self.Gr, self.Ref, self.DV, self.PV, self.Pa,

@@ -33,6 +33,17 @@ class TimeStepping:
         self.cfl_dev = torch.zeros(1, dtype=torch.float64, device=D.require_gpu())
         self._cfl_ready = False
 
+    def copy_ptr(self, PV, name) -> int:
+        if getattr(self, "copies", None) is None:
+            self.copies = torch.zeros_like(PV.values)
+        return self.copies[PV.get_nv(name)].data_ptr()
+
+    def advance(self):
+        """The time bookkeeping of update(), for a stage whose RK update ran
+        fused into the advection (pycles_advection_rk3)."""
+        if self.rk_step == self.n_rk_steps - 1:
+            self.t += self.dt
+
     def update(self, Gr, PV, Pa=None, zero_tendencies=True):
         if getattr(self, "copies", None) is None:
             self.copies = torch.zeros_like(PV.values)

@@ -59,6 +59,12 @@ cdef extern from "pycles_amd.h":
                          void *const *tend_vel, int nscal, const void *const *scal,
                          void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
                          int momentum_order, int precision, int flags, void *stream)
+    int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *ref, const void *const *vel,
+                             void *const *tend_vel, int nscal, const void *const *scal,
+                             void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
+                             int momentum_order, int precision, int flags, void *const *out_vel,
+                             void *const *out_scal, void *const *copy_vel, void *const *copy_scal,
+                             int rk_step, double dt, void *stream)
     int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt,
                          void *s, void *qt, void *wt, void *T, void *qv, void *ql, void *alpha,
                          void *buoyancy, void *theta_rho, void *bvf, void *thetali,
@@ -233,6 +239,33 @@ def advection(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase bas
     _check(pycles_advection(&dims.d, &ref.r, v3, t3 if tend_vel is not None else NULL, ns, sc, ts,
                             &base.b if base is not None else NULL, scalar_order, momentum_order,
                             precision, flags, _p(stream)))
+
+
+def advection_rk3(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase base, int scalar_order,
+                  int momentum_order, int precision, stream, int flags, out_vel, out_scal, copy_vel, copy_scal,
+                  int rk_step, double dt):
+    cdef const void *v3[3]
+    cdef void *t3[3]
+    cdef void *o3[3]
+    cdef void *c3[3]
+    cdef const void *sc[2]
+    cdef void *ts[2]
+    cdef void *osc[2]
+    cdef void *csc[2]
+    cdef int ns = len(scal)
+    for i in range(3):
+        v3[i] = _p(vel[i])
+        t3[i] = _p(tend_vel[i])
+        o3[i] = _p(out_vel[i])
+        c3[i] = _p(copy_vel[i])
+    for i in range(ns):
+        sc[i] = _p(scal[i])
+        ts[i] = _p(tend_scal[i])
+        osc[i] = _p(out_scal[i])
+        csc[i] = _p(copy_scal[i])
+    _check(pycles_advection_rk3(&dims.d, &ref.r, v3, t3, ns, sc, ts, &base.b if base is not None else NULL,
+                                scalar_order, momentum_order, precision, flags, o3, osc, c3, csc, rk_step, dt,
+                                _p(stream)))
 
 
 def thermo_sa(Dims dims, Ref ref, Lookup lt, s, qt, wt, T, qv, ql, alpha, buoyancy, theta_rho,

@@ -7,10 +7,11 @@ int adv_dispatch_f64(const Geo &g, const adv::Prof &pr, const adv::Fields &fl, i
 int adv_dispatch_f32(const Geo &g, const adv::Prof &pr, const adv::Fields &fl, int nsc, bool mom, int so, int mo,
                      hipStream_t st);
 
-extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
-                                void *const tend_vel[3], int nscal, const void *const *scal, void *const *tend_scal,
-                                const pycles_scalar_base *base, int scalar_order, int momentum_order, int precision,
-                                int flags, void *stream) {
+// Shared by both entry points; rk: optional fused RK3 stage (rk->rk >= 0).
+static int advection_run(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                         void *const tend_vel[3], int nscal, const void *const *scal, void *const *tend_scal,
+                         const pycles_scalar_base *base, int scalar_order, int momentum_order, int precision,
+                         int flags, const adv::Fields *rk, void *stream) {
     int rc = pycles_check_dims(dims);
     if (rc) return rc;
     if (!ref || !vel || !vel[0] || !vel[1] || !vel[2]) {
@@ -64,6 +65,15 @@ extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, 
         fl.q[adv::VU + d] = vel[d];
         fl.t[adv::VU + d] = mom ? tend_vel[d] : nullptr;
     }
+    fl.rk = -1;
+    if (rk) {
+        fl.rk = rk->rk;
+        fl.dt = rk->dt;
+        for (int v = 0; v < 5; ++v) {
+            fl.out[v] = rk->out[v];
+            fl.cp[v] = rk->cp[v];
+        }
+    }
     hipStream_t st = (hipStream_t)stream;
     const size_t esz = precision == PYCLES_F64 ? 8 : 4;
     if ((size_t)g.plane * g.nzg * esz >= ((size_t)1 << 32)) {
@@ -74,4 +84,58 @@ extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, 
     if (precision == PYCLES_F32) return adv_dispatch_f32(g, pr, fl, nscal, mom, so, mo, st);
     pycles_set_error("bad precision %d", precision);
     return PYCLES_ERR_ARG;
+}
+
+extern "C" int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                                void *const tend_vel[3], int nscal, const void *const *scal, void *const *tend_scal,
+                                const pycles_scalar_base *base, int scalar_order, int momentum_order, int precision,
+                                int flags, void *stream) {
+    return advection_run(dims, ref, vel, tend_vel, nscal, scal, tend_scal, base, scalar_order, momentum_order,
+                         precision, flags, nullptr, stream);
+}
+
+extern "C" int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                                    void *const tend_vel[3], int nscal, const void *const *scal,
+                                    void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,
+                                    int momentum_order, int precision, int flags, void *const out_vel[3],
+                                    void *const *out_scal, void *const copy_vel[3], void *const *copy_scal,
+                                    int rk_step, double dt, void *stream) {
+    if (rk_step < 0 || rk_step > 2) {
+        pycles_set_error("advection_rk3: rk_step %d not in [0, 2]", rk_step);
+        return PYCLES_ERR_ARG;
+    }
+    if (!tend_vel || !out_vel || !copy_vel || nscal < 0 || nscal > 2 || (nscal > 0 && (!out_scal || !copy_scal))) {
+        pycles_set_error("advection_rk3: needs the momentum and an output and copy array for every field");
+        return PYCLES_ERR_ARG;
+    }
+    if (flags & (PYCLES_ADV_PASS_X | PYCLES_ADV_PASS_Y | PYCLES_ADV_PASS_Z)) {
+        if (!(flags & PYCLES_ADV_PASS_Z)) {
+            // passes without the z pass do no stage update
+            return advection_run(dims, ref, vel, tend_vel, nscal, scal, tend_scal, base, scalar_order,
+                                 momentum_order, precision, flags, nullptr, stream);
+        }
+    }
+    adv::Fields rk;
+    memset(&rk, 0, sizeof(rk));
+    rk.rk = rk_step;
+    rk.dt = dt;
+    for (int s = 0; s < nscal; ++s) {
+        rk.out[s] = out_scal[s];
+        rk.cp[s] = copy_scal[s];
+    }
+    for (int d = 0; d < 3; ++d) {
+        rk.out[adv::VU + d] = out_vel[d];
+        rk.cp[adv::VU + d] = copy_vel[d];
+    }
+    for (int v = 0; v < 5; ++v) {
+        if (v < nscal || v >= adv::VU) {
+            const void *in = v < nscal ? scal[v] : vel[v - adv::VU];
+            if (!rk.out[v] || !rk.cp[v] || rk.out[v] == in) {
+                pycles_set_error("advection_rk3: output %d is NULL or the input array itself", v);
+                return PYCLES_ERR_ARG;
+            }
+        }
+    }
+    return advection_run(dims, ref, vel, tend_vel, nscal, scal, tend_scal, base, scalar_order, momentum_order,
+                         precision, flags, &rk, stream);
 }

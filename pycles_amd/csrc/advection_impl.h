@@ -34,6 +34,12 @@ struct Fields {
     const double *base[2];
     int tzero;  // scalar, u, v tendencies are zero on entry (x pass writes them)
     int passes;  // bit 0: x, bit 1: y, bit 2: z
+    // RK3 stage fused into the z pass (rk >= 0): new values to out, stage
+    // copies in cp, z boundary fill (sym for scalars, u, v; asym for w)
+    int rk;
+    double dt;
+    void *out[5];
+    void *cp[5];
 };
 
 struct Prof {
@@ -103,23 +109,25 @@ struct PolicyF32 {
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *bwin, TC U, double rho) {
         constexpr int R = Interp<O, float>::R;
         const bool pos = U >= 0.0f;
-        const int ctr = pos ? R - 1 : R;
-        const float pc = win[ctr];
+        // selects of values with constant indices (an index select would
+        // become a compare/select chain over the whole register window)
+        const float pc = pos ? win[R - 1] : win[R];
         float q[2 * R - 1];
         double bc;
         if (BV) {
             bc = pos ? bwin[R - 1] : bwin[R];
 #pragma unroll
             for (int m = 0; m < 2 * R - 1; ++m) {
-                const int src = pos ? m : 2 * R - 1 - m;
-                q[m] = (win[src] - pc) + (float)(bwin[src] - bc);
+                const float wv = pos ? win[m] : win[2 * R - 1 - m];
+                const double bv = pos ? bwin[m] : bwin[2 * R - 1 - m];
+                q[m] = (wv - pc) + (float)(bv - bc);
             }
         } else {
             bc = bwin[0];
 #pragma unroll
             for (int m = 0; m < 2 * R - 1; ++m) q[m] = (pos ? win[m] : win[2 * R - 1 - m]) - pc;
         }
-        const float r = pc + Interp<O, float>::weno(q);
+        const float r = pc + Interp<O, float>::weno_c0(q);
         SF32 F;
         F.f = (U * r) * (float)rho;
         F.b = ((double)U * bc) * rho;
@@ -129,11 +137,11 @@ struct PolicyF32 {
     static __device__ __forceinline__ TC mom_flux(const TC *win, TC U, double rho) {
         constexpr int R = Interp<O, float>::R;
         const bool pos = U >= 0.0f;
-        const float pc = win[pos ? R - 1 : R];
+        const float pc = pos ? win[R - 1] : win[R];
         float q[2 * R - 1];
 #pragma unroll
         for (int m = 0; m < 2 * R - 1; ++m) q[m] = (pos ? win[m] : win[2 * R - 1 - m]) - pc;
-        const float r = pc + Interp<O, float>::weno(q);
+        const float r = pc + Interp<O, float>::weno_c0(q);
         return (U * r) * (float)rho;
     }
     static __device__ __forceinline__ TC scal_div(SFlux Fc, SFlux Fp, double alpha, double dxi) {
@@ -160,6 +168,37 @@ __device__ __forceinline__ typename P::TC centred_at(const typename P::TS *f, co
 // -------------------------------------------------------------------------
 // z pass: one thread per (i, j) column, marching the faces c = k0-1 .. k1-1.
 // Block (64, 4); grid (ceil(nx/64), ceil(ny/4), nchunks).
+// The RK3 stage of one cell from its final tendency, written to fl.out[v]
+// with the z boundary fill the cell feeds (set_bcs, prognostic_variables.h:
+// 205-270): sym mirrors the first/last gw levels; asym (w) zeroes the
+// boundary level and mirrors with a sign change.
+template <typename TS>
+__device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, unsigned colo, unsigned plane, int c,
+                                         int gw, int nzg, TS vold, TS tn, bool asym) {
+    TS *cp = (TS *)fl.cp[v];
+    TS *out = (TS *)fl.out[v];
+    const TS dt = (TS)fl.dt;
+    TS vn;
+    if (fl.rk == 0) {
+        *at(cp, o) = vold;
+        vn = rk_update<TS, 0>(vold, tn, TS(0), dt);
+    } else if (fl.rk == 1) {
+        vn = rk_update<TS, 1>(vold, tn, ld(cp, o), dt);
+    } else {
+        vn = rk_update<TS, 2>(vold, tn, ld(cp, o), dt);
+    }
+    *at(out, o) = vn;
+    if (!asym) {
+        if (c < 2 * gw) *at(out, (unsigned)(2 * gw - 1 - c) * plane + colo) = vn;
+        if (c >= nzg - 2 * gw) *at(out, (unsigned)(2 * (nzg - gw) - 1 - c) * plane + colo) = vn;
+    } else {
+        if (c == gw) *at(out, (unsigned)(gw - 1) * plane + colo) = TS(0);
+        if (c <= 2 * gw - 2) *at(out, (unsigned)(2 * gw - 2 - c) * plane + colo) = -vn;
+        if (c == nzg - gw - 1) *at(out, (unsigned)(nzg - gw) * plane + colo) = TS(0);
+        if (c >= nzg - 2 * gw + 1) *at(out, (unsigned)(2 * (nzg - gw) - c) * plane + colo) = -vn;
+    }
+}
+
 template <class P, int SO, int MO, int NSC, bool MOM>
 __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchunk) {
     using TS = typename P::TS;
@@ -240,12 +279,34 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         if (c >= k0) {
             const unsigned o = lo + col;
             const double ah = pr.alpha0_half[c];
+            if (fl.rk < 0) {
 #pragma unroll
-            for (int sc = 0; sc < NSC; ++sc) *at((TS *)fl.t[sc], o) -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi);
-            if (MOM) {
-                *at((TS *)fl.t[VU], o) -= P::mom_div(Fu, Fu_prev, ah, dzi);
-                *at((TS *)fl.t[VV], o) -= P::mom_div(Fv, Fv_prev, ah, dzi);
-                *at((TS *)fl.t[VW], o) -= P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi);
+                for (int sc = 0; sc < NSC; ++sc)
+                    *at((TS *)fl.t[sc], o) -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi);
+                if (MOM) {
+                    *at((TS *)fl.t[VU], o) -= P::mom_div(Fu, Fu_prev, ah, dzi);
+                    *at((TS *)fl.t[VV], o) -= P::mom_div(Fv, Fv_prev, ah, dzi);
+                    *at((TS *)fl.t[VW], o) -= P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi);
+                }
+            } else {
+                // final tendency, as the unfused path would store it, then the stage
+#pragma unroll
+                for (int sc = 0; sc < NSC; ++sc) {
+                    TS tn = ld((const TS *)fl.t[sc], o);
+                    tn -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi);
+                    rk_store<TS>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn, false);
+                }
+                if (MOM) {
+                    TS tu = ld((const TS *)fl.t[VU], o);
+                    tu -= P::mom_div(Fu, Fu_prev, ah, dzi);
+                    rk_store<TS>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tu, false);
+                    TS tv = ld((const TS *)fl.t[VV], o);
+                    tv -= P::mom_div(Fv, Fv_prev, ah, dzi);
+                    rk_store<TS>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tv, false);
+                    TS tw = ld((const TS *)fl.t[VW], o);
+                    tw -= P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi);
+                    rk_store<TS>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tw, true);
+                }
             }
         }
 #pragma unroll

@@ -104,6 +104,26 @@ template <typename T> struct Interp<5, T> {
         const T w0 = a0 * inv, w1 = a1 * inv, w2 = a2 * inv;
         return w0 * s0 + w1 * s1 + w2 * s2;
     }
+    // weno() of a stencil whose centre value q[2] is zero (offsets from the
+    // centre): the same polynomials and smoothness indicators with the
+    // centre terms dropped.
+    static __device__ __forceinline__ T weno_c0(const T *q) {
+        const T m2 = q[0], m1 = q[1], p1 = q[3], p2 = q[4];
+        const T s0 = T(1.0 / 3.0) * m2 - T(7.0 / 6.0) * m1;
+        const T s1 = T(-1.0 / 6.0) * m1 + T(1.0 / 3.0) * p1;
+        const T s2 = T(5.0 / 6.0) * p1 - T(1.0 / 6.0) * p2;
+        const T e2 = p2 - T(2.0) * p1, f2 = p2 - T(4.0) * p1;
+        const T e1 = m1 + p1, f1 = m1 - p1;
+        const T e0 = m2 - T(2.0) * m1, f0 = m2 - T(4.0) * m1;
+        const T b2 = T(13.0 / 12.0) * e2 * e2 + T(0.25) * f2 * f2;
+        const T b1 = T(13.0 / 12.0) * e1 * e1 + T(0.25) * f1 * f1;
+        const T b0 = T(13.0 / 12.0) * e0 * e0 + T(0.25) * f0 * f0;
+        const T a0 = Num<T>::div(T(0.1), sq_eps(b0));
+        const T a1 = Num<T>::div(T(0.6), sq_eps(b1));
+        const T a2 = Num<T>::div(T(0.3), sq_eps(b2));
+        const T inv = Num<T>::rcp(a0 + a1 + a2);
+        return (a0 * s0 + a1 * s1 + a2 * s2) * inv;
+    }
     // interp_4: q = phim1, phi, phip1, phip2
     static __device__ __forceinline__ T centred(const T *q) {
         return T(7.0 / 12.0) * (q[1] + q[2]) - T(1.0 / 12.0) * (q[0] + q[3]);
@@ -139,6 +159,8 @@ template <typename T> struct Interp<7, T> {
         return w0 * s0 + w1 * s1 + w2 * s2 + w3 * s3;
     }
     // interp_6: q = phim2 .. phip3
+    // stencil with a zero centre value (no specialised form for this order)
+    static __device__ __forceinline__ T weno_c0(const T *q) { return weno(q); }
     static __device__ __forceinline__ T centred(const T *q) {
         return (T(37.0 / 60.0) * (q[2] + q[3]) - T(2.0 / 15.0) * (q[1] + q[4]) +
                 T(1.0 / 60.0) * (q[0] + q[5]));
@@ -185,6 +207,8 @@ template <typename T> struct Interp<9, T> {
         return w0 * s0 + w1 * s1 + w2 * s2 + w3 * s3 + w4 * s4;
     }
     // interp_8: q = phim3 .. phip4
+    // stencil with a zero centre value (no specialised form for this order)
+    static __device__ __forceinline__ T weno_c0(const T *q) { return weno(q); }
     static __device__ __forceinline__ T centred(const T *q) {
         return (T(533. / 840.) * (q[3] + q[4]) - T(139.0 / 840.0) * (q[2] + q[5]) +
                 T(29.0 / 840.0) * (q[1] + q[6]) - T(1.0 / 280.0) * (q[0] + q[7]));
@@ -200,6 +224,15 @@ template <typename T> struct Interp<9, T> {
 // U < 0 to the last bit: (U+|U|) = 2U and (U-|U|) = 0 exactly, 2U*phi rounds
 // to twice fl(U*phi), and the 0.5 scaling is exact.  So the kernels select the
 // upwind stencil first and evaluate one reconstruction per face.
+// TimeStepping.update_third (TimeStepping.pyx:131-151), one point of stage RK:
+// phi is the value, tn its tendency, cp the stage-0 copy.
+template <typename T, int RK>
+__device__ __forceinline__ T rk_update(T phi, T tn, T cp, T dt) {
+    if (RK == 0) return phi + tn * dt;
+    if (RK == 1) return T(0.75) * cp + T(0.25) * (phi + tn * dt);
+    return T(1.0 / 3.0) * cp + T(2.0 / 3.0) * (phi + tn * dt);
+}
+
 template <int ORDER, typename T>
 __device__ __forceinline__ T upwind_flux(const T *win, T U, T rho) {
     constexpr int R = Interp<ORDER, T>::R;

@@ -53,67 +53,151 @@ static int ilog2(long long n) {
 
 template <typename T> struct C2 { T x, y; };
 
-__device__ __forceinline__ unsigned brev(unsigned v, int bits) { return __brev(v) >> (32 - bits); }
-
-// In-place radix-2 decimation-in-time FFT over LDS arrays re/im holding
-// `batch` transforms of length N (element n of transform b at n*stride+b),
-// already in bit-reversed order.  tw[m] = exp(-2 pi i m/N), m < N/2; sign
-// +1 selects the inverse kernel (conjugate twiddles).  Butterflies are
-// evaluated as in a textbook DIT sweep: x_b <- x_a - w x_b, x_a <- x_a + w x_b.
-// batch and N are powers of two (logB = log2(batch)); tw points to LDS.
 template <typename T>
-__device__ void lds_fft(T *re, T *im, int N, int logN, int stride, int logB, const C2<T> *tw, int inverse) {
-    const int batch = 1 << logB;
-    const int nb = (N >> 1) << logB;
-    for (int s = 1; s <= logN; ++s) {
-        const int h = 1 << (s - 1);
-        const int tstep = N >> s;  // twiddle index stride
-        for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-            const int b = t & (batch - 1);
-            const int bf = t >> logB;
-            const int grp = bf >> (s - 1), pos = bf & (h - 1);
-            const int ia = (grp * 2 * h + pos) * stride + b;
-            const int ib = ia + h * stride;
-            const C2<T> w = tw[pos * tstep];
-            const T wr = w.x, wi = inverse ? -w.y : w.y;
-            const T xr = re[ib] * wr - im[ib] * wi;
-            const T xi = re[ib] * wi + im[ib] * wr;
-            const T ar = re[ia], ai = im[ia];
-            re[ib] = ar - xr;
-            im[ib] = ai - xi;
-            re[ia] = ar + xr;
-            im[ia] = ai + xi;
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+    return C2<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ void bfly(C2<T> &a, C2<T> &b) {
+    const C2<T> t = a;
+    a = C2<T>{t.x + b.x, t.y + b.y};
+    b = C2<T>{t.x - b.x, t.y - b.y};
+}
+// times -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__device__ __forceinline__ C2<T> rot90(C2<T> a) {
+    return INV ? C2<T>{-a.y, a.x} : C2<T>{a.y, -a.x};
+}
+
+// In-register DFTs of size 2, 4, 8, natural order in and out, kernel
+// exp(-+2 pi i n k / R) (forward: minus).
+template <typename T, bool INV>
+__device__ __forceinline__ void dft2(C2<T> *v) {
+    bfly(v[0], v[1]);
+}
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(C2<T> *v) {
+    bfly(v[0], v[2]);
+    bfly(v[1], v[3]);
+    v[3] = rot90<T, INV>(v[3]);
+    bfly(v[0], v[1]);
+    bfly(v[2], v[3]);
+    const C2<T> t = v[1];
+    v[1] = v[2];
+    v[2] = t;
+}
+template <typename T, bool INV>
+__device__ __forceinline__ void dft8(C2<T> *v) {
+    C2<T> e[4] = {v[0], v[2], v[4], v[6]};
+    C2<T> o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<T, INV>(e);
+    dft4<T, INV>(o);
+    const T h = T(0.70710678118654752440);
+    // o[k] *= exp(-+ i pi k / 4)
+    o[1] = INV ? C2<T>{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)}
+               : C2<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+    o[2] = rot90<T, INV>(o[2]);
+    o[3] = INV ? C2<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)}
+               : C2<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = C2<T>{e[k].x + o[k].x, e[k].y + o[k].y};
+        v[k + 4] = C2<T>{e[k].x - o[k].x, e[k].y - o[k].y};
+    }
+}
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(C2<T> *v) {
+    if (R == 2) dft2<T, INV>(v);
+    else if (R == 4) dft4<T, INV>(v);
+    else dft8<T, INV>(v);
+}
+
+// One radix-R pass of a Stockham autosort FFT over S interleaved length-N
+// sequences in LDS (element e of sequence b at buf[e * S + b]).  Thread t
+// (0 .. N/8-1) of sequence b does the 8/R butterflies q = t + s N/8: it reads
+// x[q + r N/R], twiddles by w_N^{k r N/(Ns R)} with k = q mod Ns, transforms,
+// and writes to (q - k) R + k + r Ns.  Ns is the length of the sub-transforms
+// already combined.  All threads of the block take part (two barriers).
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void stockham_pass(C2<T> *buf, int S, int b, int t, int N, int Ns, const C2<T> *tw) {
+    constexpr int NB = 8 / R;
+    const int NR = N / R, N8 = N >> 3;
+    C2<T> v[NB][R];
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+        const int q = t + s * N8;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[s][r] = buf[(q + r * NR) * S + b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NB; ++s) {
+        const int q = t + s * N8;
+        const int k = q & (Ns - 1);
+        if (Ns > 1) {
+            const int step = k * (N / (Ns * R));
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                C2<T> w = tw[step * r];
+                if (INV) w.y = -w.y;
+                v[s][r] = cmul(v[s][r], w);
+            }
         }
-        __syncthreads();
+        dft<T, R, INV>(v[s]);
+        const int base = (q - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[(base + r * Ns) * S + b] = v[s][r];
+    }
+    __syncthreads();
+}
+
+// Unnormalised DFT (forward: exp(-2 pi i n k/N); inverse: +) of the S
+// sequences in buf, N = 2^logN >= 8, in natural order in and out.
+// tw[m] = exp(-2 pi i m/N), m < N.
+template <typename T, bool INV>
+__device__ void stockham_fft(C2<T> *buf, int S, int b, int t, int N, int logN, const C2<T> *tw) {
+    int Ns = 1;
+    const int rem = logN % 3;
+    if (rem == 1) {
+        stockham_pass<T, 2, INV>(buf, S, b, t, N, Ns, tw);
+        Ns = 2;
+    } else if (rem == 2) {
+        stockham_pass<T, 4, INV>(buf, S, b, t, N, Ns, tw);
+        Ns = 4;
+    }
+    for (int p = 0; p < logN / 3; ++p) {
+        stockham_pass<T, 8, INV>(buf, S, b, t, N, Ns, tw);
+        Ns *= 8;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Level mean of w is computed before this kernel.  One block per R = 2^logR
-// consecutive rows (j0 .. j0+R-1, k'): divergence (PressureSolver.pyx:138-175,
-// zeroed then x, y, z contributions added in that order), forward FFT along x
-// of the R rows together (interleaved in LDS), half spectra written out.
+// Level mean of w is computed before this kernel.  One block per 2S
+// consecutive rows (j0 .. j0+2S-1) of one level: divergence
+// (PressureSolver.pyx:138-175, zeroed then x, y, z contributions added in that
+// order), then the forward FFT along x of the rows in pairs (row 2b as the
+// real part and row 2b+1 as the imaginary part of one complex sequence),
+// separated into the two half spectra.
 template <typename TS>
-__global__ __launch_bounds__(256) void div_fftx_kernel(Geo g, const TS *u, const TS *v, const TS *w,
+__global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const TS *v, const TS *w,
                                                        const double *alpha0, const double *alpha0_half,
                                                        const double *mean, TS *div_out, C2<TS> *spec,
-                                                       const C2<TS> *tw, int logx, int nxh, int logR) {
+                                                       const C2<TS> *tw, int logx, int nxh, int logS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int R = 1 << logR;
-    C2<TS> *twl = reinterpret_cast<C2<TS> *>(smem_raw);
-    TS *re = reinterpret_cast<TS *>(twl + g.nx / 2);
-    TS *im = re + g.nx * R;
-    for (int m = threadIdx.x; m < g.nx / 2; m += blockDim.x) twl[m] = tw[m];
+    const int S = 1 << logS;
+    C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
+    TS *bufr = reinterpret_cast<TS *>(smem_raw);
     const int kk = blockIdx.y;
     const int k = kk + g.gw;
+    const int j0 = blockIdx.x << (logS + 1);
     const double ah = alpha0_half[k];
     const double a0 = alpha0[k], a0m = alpha0[k - 1];
     const double mk = mean[k], mkm = mean[k - 1];
     const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1], dxi2 = 1.0 / g.dx[2];
-    for (int t = threadIdx.x; t < (g.nx << logR); t += blockDim.x) {
+    for (int t = threadIdx.x; t < (g.nx << (logS + 1)); t += blockDim.x) {
         const int i = t & (g.nx - 1);
         const int rb = t >> logx;
-        const int j = (blockIdx.x << logR) + rb;
+        const int j = j0 + rb;
         const int jm = j == 0 ? g.ny - 1 : j - 1;
         const int im1 = i == 0 ? g.nx - 1 : i - 1;
         const long long o = gidx(g, i, j, k);
@@ -137,56 +221,52 @@ __global__ __launch_bounds__(256) void div_fftx_kernel(Geo g, const TS *u, const
             d = (TS)dv;
         }
         if (div_out) div_out[o] = d;
-        const unsigned r = (brev((unsigned)i, logx) << logR) + rb;
-        re[r] = d;
-        im[r] = TS(0);
+        bufr[2 * (i * S + (rb >> 1)) + (rb & 1)] = d;
     }
     __syncthreads();
-    lds_fft<TS>(re, im, g.nx, logx, R, logR, twl, 0);
-    for (int rb = 0; rb < R; ++rb) {
-        const int j = (blockIdx.x << logR) + rb;
-        C2<TS> *row = spec + ((long long)kk * g.ny + j) * nxh;
-        for (int m = threadIdx.x; m < nxh; m += blockDim.x) row[m] = C2<TS>{re[(m << logR) + rb], im[(m << logR) + rb]};
+    const int b = threadIdx.x & (S - 1), t = threadIdx.x >> logS;
+    stockham_fft<TS, false>(buf, S, b, t, g.nx, logx, tw);
+    // X_a[m] = (Z[m] + conj Z[N-m]) / 2, X_b[m] = -i (Z[m] - conj Z[N-m]) / 2
+    for (int q = threadIdx.x; q < nxh * S; q += blockDim.x) {
+        const int m = q % nxh, pb = q / nxh;
+        const C2<TS> z = buf[m * S + pb];
+        const C2<TS> zm = buf[((g.nx - m) & (g.nx - 1)) * S + pb];
+        const C2<TS> xa{TS(0.5) * (z.x + zm.x), TS(0.5) * (z.y - zm.y)};
+        const C2<TS> xb{TS(0.5) * (z.y + zm.y), TS(-0.5) * (z.x - zm.x)};
+        spec[((long long)kk * g.ny + j0 + 2 * pb) * nxh + m] = xa;
+        spec[((long long)kk * g.ny + j0 + 2 * pb + 1) * nxh + m] = xb;
     }
 }
 
-// FFT along y for gx consecutive kx columns of one level; in place.
+// FFT along y for gx consecutive kx columns of one level, in place
+// (inverse: times 1/ny).
 template <typename TS>
-__global__ __launch_bounds__(256) void ffty_kernel(Geo g, C2<TS> *spec, const C2<TS> *tw, int logy, int nxh,
+__global__ __launch_bounds__(512) void ffty_kernel(Geo g, C2<TS> *spec, const C2<TS> *tw, int logy, int nxh,
                                                    int gx, int loggx, int inverse) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    C2<TS> *twl = reinterpret_cast<C2<TS> *>(smem_raw);
-    TS *re = reinterpret_cast<TS *>(twl + g.ny / 2);
-    TS *im = re + (long long)g.ny * gx;
-    for (int m = threadIdx.x; m < g.ny / 2; m += blockDim.x) twl[m] = tw[m];
+    C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
     const int kx0 = blockIdx.x * gx;
     const int kk = blockIdx.y;
     const int ncol = min(gx, nxh - kx0);
     C2<TS> *lev = spec + (long long)kk * g.ny * nxh;
-    for (int t = threadIdx.x; t < g.ny * gx; t += blockDim.x) {
-        const int y = t >> loggx, b = t & (gx - 1);
-        const unsigned r = brev((unsigned)y, logy);
-        if (b < ncol) {
-            const C2<TS> z = lev[(long long)y * nxh + kx0 + b];
-            re[r * gx + b] = z.x;
-            im[r * gx + b] = z.y;
-        } else {
-            re[r * gx + b] = TS(0);
-            im[r * gx + b] = TS(0);
-        }
+    for (int q = threadIdx.x; q < g.ny * gx; q += blockDim.x) {
+        const int y = q >> loggx, b = q & (gx - 1);
+        buf[q] = b < ncol ? lev[(long long)y * nxh + kx0 + b] : C2<TS>{TS(0), TS(0)};
     }
     __syncthreads();
-    lds_fft<TS>(re, im, g.ny, logy, gx, loggx, twl, inverse);
+    const int b = threadIdx.x & (gx - 1), t = threadIdx.x >> loggx;
+    if (inverse) stockham_fft<TS, true>(buf, gx, b, t, g.ny, logy, tw);
+    else stockham_fft<TS, false>(buf, gx, b, t, g.ny, logy, tw);
     const TS sc = inverse ? TS(1.0 / (double)g.ny) : TS(1);
-    for (int t = threadIdx.x; t < g.ny * gx; t += blockDim.x) {
-        const int y = t >> loggx, b = t & (gx - 1);
-        if (b < ncol) {
-            TS xr = re[y * gx + b], xi = im[y * gx + b];
+    for (int q = threadIdx.x; q < g.ny * gx; q += blockDim.x) {
+        const int y = q >> loggx, bb = q & (gx - 1);
+        if (bb < ncol) {
+            C2<TS> z = buf[q];
             if (inverse) {
-                xr *= sc;
-                xi *= sc;
+                z.x *= sc;
+                z.y *= sc;
             }
-            lev[(long long)y * nxh + kx0 + b] = C2<TS>{xr, xi};
+            lev[(long long)y * nxh + kx0 + bb] = z;
         }
     }
 }
@@ -198,6 +278,7 @@ template <typename TS>
 __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double *xs, double *cs,
                                                      const double *kx2, const double *ky2, const double *rho_h,
                                                      const double *dd, const double *a, const double *c, int nxh) {
+    constexpr int PF = 8;  // levels loaded ahead of the recurrence
     const long long ncol = (long long)g.ny * nxh;
     const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
@@ -209,74 +290,109 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double
         return;
     }
     const double kk = kx2[kx] + ky2[ky];
-    // forward sweep
-    C2<TS> z = spec[col];
-    double b = rho_h[0] * kk - dd[0];
-    double sc = c[0] / b;
-    double xr = (double)z.x / b, xi = (double)z.y / b;
-    cs[col] = sc;
-    xs[2 * col] = xr;
-    xs[2 * col + 1] = xi;
-    for (int k = 1; k < nz; ++k) {
-        const long long o = (long long)k * ncol + col;
-        z = spec[o];
-        b = rho_h[k] * kk - dd[k];
-        const double m = 1.0 / (b - a[k] * sc);
-        sc = c[k] * m;
-        xr = ((double)z.x - a[k] * xr) * m;
-        xi = ((double)z.y - a[k] * xi) * m;
-        cs[o] = sc;
-        xs[2 * o] = xr;
-        xs[2 * o + 1] = xi;
+    // forward sweep; the right-hand sides of the next PF levels are in flight
+    C2<TS> zq[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) zq[u] = u < nz ? spec[(long long)u * ncol + col] : C2<TS>{TS(0), TS(0)};
+    double sc = 0.0, xr = 0.0, xi = 0.0;
+    for (int k0 = 0; k0 < nz; k0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int k = k0 + u;
+            if (k < nz) {
+                const C2<TS> z = zq[u];
+                if (k + PF < nz) zq[u] = spec[(long long)(k + PF) * ncol + col];
+                const long long o = (long long)k * ncol + col;
+                const double b = rho_h[k] * kk - dd[k];
+                if (k == 0) {
+                    sc = c[0] / b;
+                    xr = (double)z.x / b;
+                    xi = (double)z.y / b;
+                } else {
+                    const double m = 1.0 / (b - a[k] * sc);
+                    sc = c[k] * m;
+                    xr = ((double)z.x - a[k] * xr) * m;
+                    xi = ((double)z.y - a[k] * xi) * m;
+                }
+                cs[o] = sc;
+                xs[2 * o] = xr;
+                xs[2 * o + 1] = xi;
+            }
+        }
     }
-    // back substitution
+    // back substitution; the scratch of the next PF levels down is in flight
     spec[(long long)(nz - 1) * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
-    for (int k = nz - 2; k >= 0; --k) {
-        const long long o = (long long)k * ncol + col;
-        const double s = cs[o];
-        xr = xs[2 * o] - s * xr;
-        xi = xs[2 * o + 1] - s * xi;
-        spec[o] = C2<TS>{(TS)xr, (TS)xi};
+    double sq[PF], rq[PF], iq[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int k = nz - 2 - u;
+        if (k >= 0) {
+            const long long o = (long long)k * ncol + col;
+            sq[u] = cs[o];
+            rq[u] = xs[2 * o];
+            iq[u] = xs[2 * o + 1];
+        }
+    }
+    for (int k0 = nz - 2; k0 >= 0; k0 -= PF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int k = k0 - u;
+            if (k >= 0) {
+                const double s_ = sq[u], r_ = rq[u], i_ = iq[u];
+                if (k - PF >= 0) {
+                    const long long on = (long long)(k - PF) * ncol + col;
+                    sq[u] = cs[on];
+                    rq[u] = xs[2 * on];
+                    iq[u] = xs[2 * on + 1];
+                }
+                xr = r_ - s_ * xr;
+                xi = i_ - s_ * xi;
+                spec[(long long)k * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
+            }
+        }
     }
 }
 
-// Inverse x transform of one row from the half spectrum (Hermitian
-// extension), real part times 1/nx, written to p at level k'+gw together with
-// its mirrored z ghost level (sym, as communicate_variable does).
+// Inverse x transform of 2S rows in pairs: Z[m] = X_a[m] + i X_b[m] with the
+// Hermitian extension X[N-m] = conj X[m], one inverse FFT, real part -> row
+// 2b and imaginary part -> row 2b+1, times 1/nx, written to p at level k'+gw
+// together with the mirrored z ghost level (sym, as communicate_variable does).
 template <typename TS>
-__global__ __launch_bounds__(256) void ifftx_kernel(Geo g, const C2<TS> *spec, TS *p, const C2<TS> *tw, int logx,
-                                                    int nxh, int logR) {
+__global__ __launch_bounds__(512) void ifftx_kernel(Geo g, const C2<TS> *spec, TS *p, const C2<TS> *tw, int logx,
+                                                    int nxh, int logS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int R = 1 << logR;
-    C2<TS> *twl = reinterpret_cast<C2<TS> *>(smem_raw);
-    TS *re = reinterpret_cast<TS *>(twl + g.nx / 2);
-    TS *im = re + g.nx * R;
-    for (int m = threadIdx.x; m < g.nx / 2; m += blockDim.x) twl[m] = tw[m];
+    const int S = 1 << logS;
+    C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
+    const TS *bufr = reinterpret_cast<const TS *>(smem_raw);
     const int kk = blockIdx.y;
-    for (int t = threadIdx.x; t < (g.nx << logR); t += blockDim.x) {
-        const int m = t & (g.nx - 1);
-        const int rb = t >> logx;
-        const int j = (blockIdx.x << logR) + rb;
-        const C2<TS> *row = spec + ((long long)kk * g.ny + j) * nxh;
-        C2<TS> z;
-        if (m < nxh) z = row[m];
-        else {
-            z = row[g.nx - m];
-            z.y = -z.y;
+    const int j0 = blockIdx.x << (logS + 1);
+    for (int q = threadIdx.x; q < (g.nx << logS); q += blockDim.x) {
+        const int m = q & (g.nx - 1);
+        const int pb = q >> logx;
+        const C2<TS> *ra = spec + ((long long)kk * g.ny + j0 + 2 * pb) * nxh;
+        const C2<TS> *rb = ra + nxh;
+        C2<TS> xa, xb;
+        if (m < nxh) {
+            xa = ra[m];
+            xb = rb[m];
+        } else {
+            xa = ra[g.nx - m];
+            xb = rb[g.nx - m];
+            xa.y = -xa.y;
+            xb.y = -xb.y;
         }
-        const unsigned r = (brev((unsigned)m, logx) << logR) + rb;
-        re[r] = z.x;
-        im[r] = z.y;
+        buf[m * S + pb] = C2<TS>{xa.x - xb.y, xa.y + xb.x};
     }
     __syncthreads();
-    lds_fft<TS>(re, im, g.nx, logx, R, logR, twl, 1);
+    const int b = threadIdx.x & (S - 1), t = threadIdx.x >> logS;
+    stockham_fft<TS, true>(buf, S, b, t, g.nx, logx, tw);
     const TS sc = TS(1.0 / (double)g.nx);
     const int k = kk + g.gw;
-    for (int t = threadIdx.x; t < (g.nx << logR); t += blockDim.x) {
-        const int i = t & (g.nx - 1);
-        const int rb = t >> logx;
-        const int j = (blockIdx.x << logR) + rb;
-        const TS val = re[(i << logR) + rb] * sc;
+    for (int q = threadIdx.x; q < (g.nx << (logS + 1)); q += blockDim.x) {
+        const int i = q & (g.nx - 1);
+        const int rb = q >> logx;
+        const int j = j0 + rb;
+        const TS val = bufr[2 * (i * S + (rb >> 1)) + (rb & 1)] * sc;
         p[gidx(g, i, j, k)] = val;
         if (kk < g.gw) p[gidx(g, i, j, g.gw - 1 - kk)] = val;
         if (kk >= g.nz - g.gw) p[gidx(g, i, j, 2 * g.gw + 2 * g.nz - 1 - k)] = val;
@@ -383,8 +499,8 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     }
     Geo g = make_geo(dims);
     const int logx = ilog2(g.nx), logy = ilog2(g.ny);
-    if (logx < 1 || logy < 1 || g.nx > 4096 || g.ny > 4096) {
-        pycles_set_error("pressure: nx, ny must be powers of two in [2, 4096] (got %d, %d)", g.nx, g.ny);
+    if (logx < 3 || logy < 3 || g.nx > 4096 || g.ny > 4096) {
+        pycles_set_error("pressure: nx, ny must be powers of two in [8, 4096] (got %d, %d)", g.nx, g.ny);
         return PYCLES_ERR_UNSUPPORTED;
     }
     if (g.nz < 2) {
@@ -400,8 +516,9 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     P->logy = logy;
     const size_t cs = precision == PYCLES_F64 ? 16 : 8;
     const size_t es = cs / 2;
+    // y transform: gx kx columns per block, at most 512 threads and 64 KB
     int gx = 16;
-    while (gx > 1 && (size_t)2 * g.ny * gx * es + (size_t)g.ny / 2 * cs > 64 * 1024) gx >>= 1;
+    while (gx > 1 && ((size_t)gx * g.ny / 8 > 512 || (size_t)gx * g.ny * cs > 64 * 1024)) gx >>= 1;
     P->gx = gx;
     P->loggx = ilog2(gx);
     const long long nspec = (long long)g.nz * g.ny * P->nxh;
@@ -427,13 +544,14 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
         a[k] = (k == 0) ? 0.0 : dzi * dzi * rho0[k + gw - 1];
         c[k] = (k == nz - 1) ? 0.0 : dzi * dzi * rho0[k + gw];
     }
-    std::vector<double> twx(g.nx), twy(g.ny);
-    for (int m = 0; m < g.nx / 2; ++m) {
+    // twiddles exp(-2 pi i m / N), m < N, interleaved (re, im)
+    std::vector<double> twx(2 * g.nx), twy(2 * g.ny);
+    for (int m = 0; m < g.nx; ++m) {
         const double ang = -2.0 * PC_PI * (double)m / (double)g.nx;
         twx[2 * m] = cos(ang);
         twx[2 * m + 1] = sin(ang);
     }
-    for (int m = 0; m < g.ny / 2; ++m) {
+    for (int m = 0; m < g.ny; ++m) {
         const double ang = -2.0 * PC_PI * (double)m / (double)g.ny;
         twy[2 * m] = cos(ang);
         twy[2 * m + 1] = sin(ang);
@@ -449,8 +567,8 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     ALLOC(P->spec, nspec * cs);
     ALLOC(P->xs, nspec * 16);
     ALLOC(P->cs, nspec * 8);
-    ALLOC(P->twx, (size_t)g.nx / 2 * cs + cs);
-    ALLOC(P->twy, (size_t)g.ny / 2 * cs + cs);
+    ALLOC(P->twx, (size_t)g.nx * cs);
+    ALLOC(P->twy, (size_t)g.ny * cs);
     ALLOC(P->kx2, P->nxh * 8);
     ALLOC(P->ky2, g.ny * 8);
     ALLOC(P->rho_h, nz * 8);
@@ -463,14 +581,14 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
 #undef ALLOC
     hipError_t e = hipSuccess;
     if (precision == PYCLES_F64) {
-        e = hipMemcpy(P->twx, twx.data(), g.nx / 2 * 16, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(P->twy, twy.data(), g.ny / 2 * 16, hipMemcpyHostToDevice);
+        e = hipMemcpy(P->twx, twx.data(), (size_t)g.nx * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->twy, twy.data(), (size_t)g.ny * 16, hipMemcpyHostToDevice);
     } else {
-        std::vector<float> fx(g.nx), fy(g.ny);
-        for (int m = 0; m < g.nx; ++m) fx[m] = (float)twx[m];
-        for (int m = 0; m < g.ny; ++m) fy[m] = (float)twy[m];
-        e = hipMemcpy(P->twx, fx.data(), g.nx / 2 * 8, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(P->twy, fy.data(), g.ny / 2 * 8, hipMemcpyHostToDevice);
+        std::vector<float> fx(2 * g.nx), fy(2 * g.ny);
+        for (int m = 0; m < 2 * g.nx; ++m) fx[m] = (float)twx[m];
+        for (int m = 0; m < 2 * g.ny; ++m) fy[m] = (float)twy[m];
+        e = hipMemcpy(P->twx, fx.data(), (size_t)g.nx * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(P->twy, fy.data(), (size_t)g.ny * 8, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess) e = hipMemcpy(P->kx2, kx2.data(), P->nxh * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(P->ky2, ky2.data(), g.ny * 8, hipMemcpyHostToDevice);
@@ -502,30 +620,31 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
     const Geo &g = P->g;
     int rc = pycles_level_mean_ws(g, w, P->mean, P->partial, nullptr, P->precision, st);
     if (rc) return rc;
-    // rows per block for the x transforms: about 2048 points per block
-    int logR = 0;
-    while ((g.nx << (logR + 1)) <= 2048 && (1 << (logR + 1)) <= g.ny && logR < 4) ++logR;
-    const int R = 1 << logR;
-    const size_t rowsmem = 2 * sizeof(TS) * g.nx * R + sizeof(C2<TS>) * (g.nx / 2);
-    const int bx = 256;
-    hipLaunchKernelGGL(div_fftx_kernel<TS>, dim3(g.ny / R, g.nz), dim3(bx), rowsmem, st, g, (const TS *)u,
+    // x transforms: S complex sequences (2S rows) per block, S nx/8 threads
+    int logS = 0;
+    while ((g.nx << (logS + 1)) <= 2048 && (2 << (logS + 1)) <= g.ny) ++logS;
+    const int S = 1 << logS;
+    const size_t rowsmem = sizeof(C2<TS>) * (size_t)g.nx * S;
+    const int bx = S * g.nx / 8;
+    hipLaunchKernelGGL(div_fftx_kernel<TS>, dim3(g.ny / (2 * S), g.nz), dim3(bx), rowsmem, st, g, (const TS *)u,
                        (const TS *)v, (const TS *)w, ref->alpha0, ref->alpha0_half, P->mean, (TS *)div,
-                       (C2<TS> *)P->spec, (const C2<TS> *)P->twx, P->logx, P->nxh, logR);
+                       (C2<TS> *)P->spec, (const C2<TS> *)P->twx, P->logx, P->nxh, logS);
     PYCLES_CHECK_LAUNCH();
-    const size_t ysmem = 2 * sizeof(TS) * (size_t)g.ny * P->gx + sizeof(C2<TS>) * (g.ny / 2);
+    const size_t ysmem = sizeof(C2<TS>) * (size_t)g.ny * P->gx;
     const dim3 ygrid((P->nxh + P->gx - 1) / P->gx, g.nz);
-    hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(256), ysmem, st, g, (C2<TS> *)P->spec,
+    const int by = P->gx * g.ny / 8;
+    hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 0);
     PYCLES_CHECK_LAUNCH();
     const long long ncol = (long long)g.ny * P->nxh;
     hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
                        (C2<TS> *)P->spec, P->xs, P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c, P->nxh);
     PYCLES_CHECK_LAUNCH();
-    hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(256), ysmem, st, g, (C2<TS> *)P->spec,
+    hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 1);
     PYCLES_CHECK_LAUNCH();
-    hipLaunchKernelGGL(ifftx_kernel<TS>, dim3(g.ny / R, g.nz), dim3(bx), rowsmem, st, g, (const C2<TS> *)P->spec,
-                       (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, logR);
+    hipLaunchKernelGGL(ifftx_kernel<TS>, dim3(g.ny / (2 * S), g.nz), dim3(bx), rowsmem, st, g,
+                       (const C2<TS> *)P->spec, (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, logS);
     PYCLES_CHECK_LAUNCH();
     if (cfl_out) hipLaunchKernelGGL(cfl_zero_kernel, dim3(1), dim3(1), 0, st, P->cfl);
     hipLaunchKernelGGL(correct_kernel<TS>, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, st, g, (TS *)u, (TS *)v,

@@ -85,12 +85,6 @@ struct PtrPack {
 // One thread per column: interior update in k, then the ghost fill of the
 // same column, so no thread reads a level another thread writes.
 // Grid (columns/256, nv): blockIdx.y selects the variable.
-template <typename T, int RK>
-__device__ __forceinline__ T rk_update(T phi, T tn, T cp, T dt) {
-    if (RK == 0) return phi + tn * dt;
-    if (RK == 1) return T(0.75) * cp + T(0.25) * (phi + tn * dt);
-    return T(1.0 / 3.0) * cp + T(2.0 / 3.0) * (phi + tn * dt);
-}
 
 template <typename T, int RK>
 __global__ __launch_bounds__(256) void rk3_kernel(Geo g, PtrPack pk, double dtd, int zero_tend) {
