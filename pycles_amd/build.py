@@ -58,8 +58,11 @@ def build_hip(force=True, jobs=8):
         if force or _stale(obj, [src] + headers):
             flags = list(HIP_FLAGS)
             if s.endswith("_f32.hip"):
-                # FP32-only translation units fuse multiply-adds
-                flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast"
+                # FP32-only translation units fuse multiply-adds, except inside
+                # `#pragma clang fp contract(off)` regions (the RK stage
+                # update shares its rounding with the separate RK kernel);
+                # plain `fast` ignores that pragma in the backend
+                flags[flags.index("-ffp-contract=off")] = "-ffp-contract=fast-honor-pragmas"
             jobs_list.append([HIPCC] + flags + ["-c", src, "-o", obj])
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, jobs_list))

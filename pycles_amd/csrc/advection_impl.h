@@ -279,32 +279,32 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         if (c >= k0) {
             const unsigned o = lo + col;
             const double ah = pr.alpha0_half[c];
+            // the final tendencies, with an uncontracted subtraction so the
+            // stored (separate RK3) and fused paths round identically
+            TS tsc[NSC > 0 ? NSC : 1], tu = 0, tv = 0, tw = 0;
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc)
+                tsc[sc] = add_rn(ld((const TS *)fl.t[sc], o), -P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi));
+            if (MOM) {
+                tu = add_rn(ld((const TS *)fl.t[VU], o), -P::mom_div(Fu, Fu_prev, ah, dzi));
+                tv = add_rn(ld((const TS *)fl.t[VV], o), -P::mom_div(Fv, Fv_prev, ah, dzi));
+                tw = add_rn(ld((const TS *)fl.t[VW], o), -P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
+            }
             if (fl.rk < 0) {
 #pragma unroll
-                for (int sc = 0; sc < NSC; ++sc)
-                    *at((TS *)fl.t[sc], o) -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi);
+                for (int sc = 0; sc < NSC; ++sc) *at((TS *)fl.t[sc], o) = tsc[sc];
                 if (MOM) {
-                    *at((TS *)fl.t[VU], o) -= P::mom_div(Fu, Fu_prev, ah, dzi);
-                    *at((TS *)fl.t[VV], o) -= P::mom_div(Fv, Fv_prev, ah, dzi);
-                    *at((TS *)fl.t[VW], o) -= P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi);
+                    *at((TS *)fl.t[VU], o) = tu;
+                    *at((TS *)fl.t[VV], o) = tv;
+                    *at((TS *)fl.t[VW], o) = tw;
                 }
             } else {
-                // final tendency, as the unfused path would store it, then the stage
 #pragma unroll
-                for (int sc = 0; sc < NSC; ++sc) {
-                    TS tn = ld((const TS *)fl.t[sc], o);
-                    tn -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi);
-                    rk_store<TS>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn, false);
-                }
+                for (int sc = 0; sc < NSC; ++sc)
+                    rk_store<TS>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tsc[sc], false);
                 if (MOM) {
-                    TS tu = ld((const TS *)fl.t[VU], o);
-                    tu -= P::mom_div(Fu, Fu_prev, ah, dzi);
                     rk_store<TS>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tu, false);
-                    TS tv = ld((const TS *)fl.t[VV], o);
-                    tv -= P::mom_div(Fv, Fv_prev, ah, dzi);
                     rk_store<TS>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tv, false);
-                    TS tw = ld((const TS *)fl.t[VW], o);
-                    tw -= P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi);
                     rk_store<TS>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tw, true);
                 }
             }

@@ -224,13 +224,36 @@ template <typename T> struct Interp<9, T> {
 // U < 0 to the last bit: (U+|U|) = 2U and (U-|U|) = 0 exactly, 2U*phi rounds
 // to twice fl(U*phi), and the 0.5 scaling is exact.  So the kernels select the
 // upwind stencil first and evaluate one reconstruction per face.
+// Correctly rounded multiply and add that are never contracted into an FMA
+// (the pragma drops the contraction permission from these operations), so an
+// expression gives the same bits in every translation unit whatever its
+// contraction flag.
+__device__ __forceinline__ double mul_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double add_rn(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 // TimeStepping.update_third (TimeStepping.pyx:131-151), one point of stage RK:
-// phi is the value, tn its tendency, cp the stage-0 copy.
+// phi is the value, tn its tendency, cp the stage-0 copy.  Uncontracted, in
+// the reference's operation order.
 template <typename T, int RK>
 __device__ __forceinline__ T rk_update(T phi, T tn, T cp, T dt) {
-    if (RK == 0) return phi + tn * dt;
-    if (RK == 1) return T(0.75) * cp + T(0.25) * (phi + tn * dt);
-    return T(1.0 / 3.0) * cp + T(2.0 / 3.0) * (phi + tn * dt);
+    const T x = add_rn(phi, mul_rn(tn, dt));
+    if (RK == 0) return x;
+    if (RK == 1) return add_rn(mul_rn(T(0.75), cp), mul_rn(T(0.25), x));
+    return add_rn(mul_rn(T(1.0 / 3.0), cp), mul_rn(T(2.0 / 3.0), x));
 }
 
 template <int ORDER, typename T>

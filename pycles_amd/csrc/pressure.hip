@@ -241,7 +241,7 @@ __global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const
 // FFT along y for gx consecutive kx columns of one level, in place
 // (inverse: times 1/ny).
 template <typename TS>
-__global__ __launch_bounds__(512) void ffty_kernel(Geo g, C2<TS> *spec, const C2<TS> *tw, int logy, int nxh,
+__global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C2<TS> *tw, int logy, int nxh,
                                                    int gx, int loggx, int inverse) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
@@ -404,19 +404,25 @@ __global__ __launch_bounds__(512) void ifftx_kernel(Geo g, const C2<TS> *spec, T
 // column, and the CFL maximum of the corrected interior.
 template <typename TS>
 __global__ __launch_bounds__(256) void correct_kernel(Geo g, TS *u, TS *v, TS *w, const TS *p, const double *mean,
-                                                      double dt, double *cfl) {
+                                                      double dt, double *cfl, int kchunk) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
+    const int k0 = g.gw + blockIdx.z * kchunk;
+    const int k1 = min(k0 + kchunk, g.nz + g.gw);
     double cm = 0.0;
     if (i < g.nx) {
         const int ip = i + 1 == g.nx ? 0 : i + 1;
         const int jp = j + 1 == g.ny ? 0 : j + 1;
         const long long col = (long long)j * g.nx + i;
-        for (int k = g.gw; k < g.nz + g.gw; ++k) {
-            const long long o = (long long)k * g.plane + col;
-            const TS pc = p[o];
-            const TS pe = p[gidx(g, ip, j, k)];
-            const TS pn = p[gidx(g, i, jp, k)];
+        const long long ce = (long long)j * g.nx + ip;
+        const long long cn = (long long)jp * g.nx + i;
+        TS pc = p[(long long)k0 * g.plane + col];
+#pragma unroll 4
+        for (int k = k0; k < k1; ++k) {
+            const long long lo = (long long)k * g.plane;
+            const long long o = lo + col;
+            const TS pe = p[lo + ce];
+            const TS pn = p[lo + cn];
             const TS pt = p[o + g.plane];
             TS un, vn, wn;
             if (sizeof(TS) == 8) {
@@ -437,18 +443,26 @@ __global__ __launch_bounds__(256) void correct_kernel(Geo g, TS *u, TS *v, TS *w
                                        fabs((double)wn) * g.dxi[2]);
                 cm = fmax(cm, c);
             }
+            pc = pt;
         }
-        for (int m = 0; m < g.gw; ++m) {
-            u[gidx(g, i, j, g.gw - 1 - m)] = u[gidx(g, i, j, g.gw + m)];
-            u[gidx(g, i, j, g.nzg - g.gw + m)] = u[gidx(g, i, j, g.nzg - g.gw - 1 - m)];
-            v[gidx(g, i, j, g.gw - 1 - m)] = v[gidx(g, i, j, g.gw + m)];
-            v[gidx(g, i, j, g.nzg - g.gw + m)] = v[gidx(g, i, j, g.nzg - g.gw - 1 - m)];
+        // z ghost levels from the corrected boundary levels this thread wrote
+        // (chunks hold at least 2 gw levels)
+        if (k0 == g.gw) {
+            for (int m = 0; m < g.gw; ++m) {
+                u[gidx(g, i, j, g.gw - 1 - m)] = u[gidx(g, i, j, g.gw + m)];
+                v[gidx(g, i, j, g.gw - 1 - m)] = v[gidx(g, i, j, g.gw + m)];
+            }
+            w[gidx(g, i, j, g.gw - 1)] = TS(0);
+            for (int m = 1; m < g.gw; ++m) w[gidx(g, i, j, g.gw - 1 - m)] = TS(-1.0) * w[gidx(g, i, j, g.gw - 1 + m)];
         }
-        w[gidx(g, i, j, g.gw - 1)] = TS(0);
-        w[gidx(g, i, j, g.nzg - g.gw)] = TS(0);
-        for (int m = 1; m < g.gw; ++m) {
-            w[gidx(g, i, j, g.gw - 1 - m)] = TS(-1.0) * w[gidx(g, i, j, g.gw - 1 + m)];
-            w[gidx(g, i, j, g.nzg - g.gw + m)] = TS(-1.0) * w[gidx(g, i, j, g.nzg - g.gw - m)];
+        if (k1 == g.nz + g.gw) {
+            for (int m = 0; m < g.gw; ++m) {
+                u[gidx(g, i, j, g.nzg - g.gw + m)] = u[gidx(g, i, j, g.nzg - g.gw - 1 - m)];
+                v[gidx(g, i, j, g.nzg - g.gw + m)] = v[gidx(g, i, j, g.nzg - g.gw - 1 - m)];
+            }
+            w[gidx(g, i, j, g.nzg - g.gw)] = TS(0);
+            for (int m = 1; m < g.gw; ++m)
+                w[gidx(g, i, j, g.nzg - g.gw + m)] = TS(-1.0) * w[gidx(g, i, j, g.nzg - g.gw - m)];
         }
     }
     if (cfl) {
@@ -516,9 +530,10 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     P->logy = logy;
     const size_t cs = precision == PYCLES_F64 ? 16 : 8;
     const size_t es = cs / 2;
-    // y transform: gx kx columns per block, at most 512 threads and 64 KB
+    // y transform: gx kx columns per block (16 makes 128-byte row segments in
+    // FP32), at most 1024 threads and 64 KB
     int gx = 16;
-    while (gx > 1 && ((size_t)gx * g.ny / 8 > 512 || (size_t)gx * g.ny * cs > 64 * 1024)) gx >>= 1;
+    while (gx > 1 && ((size_t)gx * g.ny / 8 > 1024 || (size_t)gx * g.ny * cs > 64 * 1024)) gx >>= 1;
     P->gx = gx;
     P->loggx = ilog2(gx);
     const long long nspec = (long long)g.nz * g.ny * P->nxh;
@@ -647,8 +662,12 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
                        (const C2<TS> *)P->spec, (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, logS);
     PYCLES_CHECK_LAUNCH();
     if (cfl_out) hipLaunchKernelGGL(cfl_zero_kernel, dim3(1), dim3(1), 0, st, P->cfl);
-    hipLaunchKernelGGL(correct_kernel<TS>, dim3((g.nx + 255) / 256, g.ny), dim3(256), 0, st, g, (TS *)u, (TS *)v,
-                       (TS *)w, (const TS *)p, P->mean, dt, cfl_out ? P->cfl : nullptr);
+    // z chunks (at least 2 gw levels each) so the grid holds ~16k waves
+    int nch = 1;
+    while ((long long)g.plane / 64 * nch < 16384 && g.nz / (2 * nch) >= 2 * g.gw && nch < 16) nch *= 2;
+    const int kchunk = (g.nz + nch - 1) / nch;
+    hipLaunchKernelGGL(correct_kernel<TS>, dim3((g.nx + 255) / 256, g.ny, nch), dim3(256), 0, st, g, (TS *)u,
+                       (TS *)v, (TS *)w, (const TS *)p, P->mean, dt, cfl_out ? P->cfl : nullptr, kchunk);
     PYCLES_CHECK_LAUNCH();
     if (cfl_out) {
         hipLaunchKernelGGL(cfl_add_kernel, dim3(1), dim3(1), 0, st, P->cfl, cfl_out);

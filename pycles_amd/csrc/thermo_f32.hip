@@ -52,6 +52,9 @@ __global__ __launch_bounds__(256, 4) void thermo_sa_f32_kernel(Geo g, SAArgs a) 
     const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
     const float dzi = (float)(1.0 / g.dx[2]);
     float Tw[3] = {0, 0, 0}, qtw[3] = {0, 0, 0}, qvw[3] = {0, 0, 0}, thw[3] = {0, 0, 0}, bprev = 0.0f;
+    // s and qt of the next level are loaded one iteration ahead
+    float s_nx = active ? s[(long long)ch.kw * g.plane + col] : 0.0f;
+    float q_nx = active ? qtp[(long long)ch.kw * g.plane + col] : 0.0f;
     for (int k = ch.kw; k < ch.ke; ++k) {
         const bool own = active && k >= ch.k0 && k < ch.k1;
         const bool own_m = active && k - 1 >= ch.k0 && k - 1 < ch.k1;
@@ -60,8 +63,12 @@ __global__ __launch_bounds__(256, 4) void thermo_sa_f32_kernel(Geo g, SAArgs a) 
         const float p0f = (float)p0;
         const double sb = a.sbase ? a.sbase[k] : 0.0;
         const double qb = a.qbase ? a.qbase[k] : 0.0;
-        float sd = active ? s[o] : 0.0f;
-        float qd = active ? qtp[o] : 0.0f;
+        float sd = s_nx;
+        float qd = q_nx;
+        if (active && k + 1 < ch.ke) {
+            s_nx = s[o + g.plane];
+            q_nx = qtp[o + g.plane];
+        }
         double qt64 = qb + (double)qd;
         if (a.clip && !(qt64 >= 1e-11)) {
             qt64 = 1e-11;

@@ -176,8 +176,10 @@ def test_rk3_stage_and_bcs_bitwise(gpu, rk):
     assert float(sim.PV.tendencies.abs().max()) == 0.0
 
 
-def test_pressure_parity(gpu):
-    nl, Gr, RS, Th, fields = host_case(32, 16, 24, seed=9)
+@pytest.mark.parametrize("nx,ny", [(32, 16), (8, 64), (16, 128), (64, 8), (256, 32)])
+def test_pressure_parity(gpu, nx, ny):
+    # nx, ny cover first Stockham passes of radix 2, 4 and 8 in both directions
+    nl, Gr, RS, Th, fields = host_case(nx, ny, 24, seed=9)
     fields = _swirl(Gr, fields, amp=2.0)
     sim = _gpu_sim(nl, Gr, RS, Th, fields)
     sim.Pr.update(sim.Gr, sim.Ref, sim.DV, sim.PV, sim.Pa)
