@@ -78,7 +78,8 @@ struct PolicyF64 {
         return upwind_flux<O, double>(win, U, rho);
     }
     // tend -= alpha*(Fc - Fp)*dxi, evaluated as the reference does.
-    static __device__ __forceinline__ TC scal_div(SFlux Fc, SFlux Fp, double alpha, double dxi) {
+    template <bool BV>
+    static __device__ __forceinline__ TC scal_div(SFlux Fc, SFlux Fp, double alpha, double dxi, double) {
         return alpha * (Fc - Fp) * dxi;
     }
     static __device__ __forceinline__ TC mom_div(TC Fc, TC Fp, double alpha, double dxi) {
@@ -95,7 +96,10 @@ struct PolicyF64 {
 // pair: the small part rho*U*(phi'_ctr + R(d)) in FP32 and the base part
 // rho*U*base_ctr in FP64, and the base part's flux difference is formed in
 // FP64 before the tendency is rounded to FP32 (compensated flux divergence).
-// BV: the base varies along the line (z); in x and y it is one value.
+// BV: the base varies along the line (z); in x and y it is one value bc, and
+// the pair carries U itself: the base part's flux difference is then
+// (U_c - U_{c-1}) * bc * rho, with the difference of two FP32 values exact
+// in FP64.
 struct SF32 {
     float f;
     double b;
@@ -113,7 +117,7 @@ struct PolicyF32 {
         // become a compare/select chain over the whole register window)
         const float pc = pos ? win[R - 1] : win[R];
         float q[2 * R - 1];
-        double bc;
+        double bc = 0.0;
         if (BV) {
             bc = pos ? bwin[R - 1] : bwin[R];
 #pragma unroll
@@ -123,14 +127,13 @@ struct PolicyF32 {
                 q[m] = (wv - pc) + (float)(bv - bc);
             }
         } else {
-            bc = bwin[0];
 #pragma unroll
             for (int m = 0; m < 2 * R - 1; ++m) q[m] = (pos ? win[m] : win[2 * R - 1 - m]) - pc;
         }
         const float r = pc + Interp<O, float>::weno_c0(q);
         SF32 F;
         F.f = (U * r) * (float)rho;
-        F.b = ((double)U * bc) * rho;
+        F.b = BV ? ((double)U * bc) * rho : (double)U;
         return F;
     }
     template <int O>
@@ -144,9 +147,11 @@ struct PolicyF32 {
         const float r = pc + Interp<O, float>::weno_c0(q);
         return (U * r) * (float)rho;
     }
-    static __device__ __forceinline__ TC scal_div(SFlux Fc, SFlux Fp, double alpha, double dxi) {
-        const double big = Fc.b - Fp.b;
+    // brho = bc * rho for a line of constant base (BV false), unused otherwise
+    template <bool BV>
+    static __device__ __forceinline__ TC scal_div(SFlux Fc, SFlux Fp, double alpha, double dxi, double brho) {
         const float small = Fc.f - Fp.f;
+        const double big = BV ? (Fc.b - Fp.b) : (Fc.b - Fp.b) * brho;
         return (float)((big + (double)small) * (alpha * dxi));
     }
     static __device__ __forceinline__ TC mom_div(TC Fc, TC Fp, double alpha, double dxi) {
@@ -172,20 +177,21 @@ __device__ __forceinline__ typename P::TC centred_at(const typename P::TS *f, co
 // with the z boundary fill the cell feeds (set_bcs, prognostic_variables.h:
 // 205-270): sym mirrors the first/last gw levels; asym (w) zeroes the
 // boundary level and mirrors with a sign change.
+// cpv is the stage copy at o (read by the caller; unused in stage 0, which
+// writes it).
 template <typename TS>
 __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, unsigned colo, unsigned plane, int c,
-                                         int gw, int nzg, TS vold, TS tn, bool asym) {
-    TS *cp = (TS *)fl.cp[v];
+                                         int gw, int nzg, TS vold, TS tn, TS cpv, bool asym) {
     TS *out = (TS *)fl.out[v];
     const TS dt = (TS)fl.dt;
     TS vn;
     if (fl.rk == 0) {
-        *at(cp, o) = vold;
+        *at((TS *)fl.cp[v], o) = vold;
         vn = rk_update<TS, 0>(vold, tn, TS(0), dt);
     } else if (fl.rk == 1) {
-        vn = rk_update<TS, 1>(vold, tn, ld(cp, o), dt);
+        vn = rk_update<TS, 1>(vold, tn, cpv, dt);
     } else {
-        vn = rk_update<TS, 2>(vold, tn, ld(cp, o), dt);
+        vn = rk_update<TS, 2>(vold, tn, cpv, dt);
     }
     *at(out, o) = vn;
     if (!asym) {
@@ -207,10 +213,11 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int NC = 2 * (RM - 1);
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
+    const uint3 tile = xcd_tile<1>();  // XCD bands of 4-row blocks in y
+    const int i = tile.x * 64 + threadIdx.x;
+    const int j = tile.y * 4 + threadIdx.y;
     if (i >= g.nx || j >= g.ny) return;  // no barriers in this kernel
-    const int k0 = g.gw + blockIdx.z * kchunk;
+    const int k0 = g.gw + tile.z * kchunk;
     const int k1 = min(k0 + kchunk, g.nz + g.gw);
     const unsigned plane = (unsigned)g.plane;
     const unsigned col = (unsigned)j * g.nx + i;
@@ -253,6 +260,22 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
 
     for (; c < k1; ++c) {
         const unsigned lo = (unsigned)c * plane;
+        const unsigned o = lo + col;
+        const bool upd = c >= k0;
+        // This cell's tendencies (and stage copies) are read before the flux
+        // arithmetic: the compiler must assume the output arrays alias them,
+        // so a read placed next to its use could not move above the previous
+        // field's store, and each field would cost one memory round trip.
+        TS told[5] = {}, cpv[5] = {};
+        if (upd) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (v < NSC || (MOM && v >= VU)) {
+                    told[v] = ld((const TS *)fl.t[v], o);
+                    if (fl.rk > 0) cpv[v] = ld((const TS *)fl.cp[v], o);
+                }
+            }
+        }
         const double rho0c = pr.rho0[c];
         const TC W = MOM ? wsw[RM - 1] : (TC)ld(pw, lo + col);
         SFlux Fs[NSC > 0 ? NSC : 1];
@@ -276,36 +299,31 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
             for (int m = 0; m < NC; ++m) q[m] = wsw[m + 1];
             Fw = P::template mom_flux<MO>(wsw, Interp<MO, TC>::centred(q), pr.rho0_half[c + 1]);
         }
-        if (c >= k0) {
-            const unsigned o = lo + col;
+        if (upd) {
             const double ah = pr.alpha0_half[c];
             // the final tendencies, with an uncontracted subtraction so the
             // stored (separate RK3) and fused paths round identically
-            TS tsc[NSC > 0 ? NSC : 1], tu = 0, tv = 0, tw = 0;
+            TS tn[5] = {};
 #pragma unroll
             for (int sc = 0; sc < NSC; ++sc)
-                tsc[sc] = add_rn(ld((const TS *)fl.t[sc], o), -P::scal_div(Fs[sc], Fs_prev[sc], ah, dzi));
+                tn[sc] = add_rn(told[sc], -P::template scal_div<true>(Fs[sc], Fs_prev[sc], ah, dzi, 1.0));
             if (MOM) {
-                tu = add_rn(ld((const TS *)fl.t[VU], o), -P::mom_div(Fu, Fu_prev, ah, dzi));
-                tv = add_rn(ld((const TS *)fl.t[VV], o), -P::mom_div(Fv, Fv_prev, ah, dzi));
-                tw = add_rn(ld((const TS *)fl.t[VW], o), -P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
+                tn[VU] = add_rn(told[VU], -P::mom_div(Fu, Fu_prev, ah, dzi));
+                tn[VV] = add_rn(told[VV], -P::mom_div(Fv, Fv_prev, ah, dzi));
+                tn[VW] = add_rn(told[VW], -P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
             }
             if (fl.rk < 0) {
 #pragma unroll
-                for (int sc = 0; sc < NSC; ++sc) *at((TS *)fl.t[sc], o) = tsc[sc];
-                if (MOM) {
-                    *at((TS *)fl.t[VU], o) = tu;
-                    *at((TS *)fl.t[VV], o) = tv;
-                    *at((TS *)fl.t[VW], o) = tw;
-                }
+                for (int v = 0; v < 5; ++v)
+                    if (v < NSC || (MOM && v >= VU)) *at((TS *)fl.t[v], o) = tn[v];
             } else {
 #pragma unroll
                 for (int sc = 0; sc < NSC; ++sc)
-                    rk_store<TS>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tsc[sc], false);
+                    rk_store<TS>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn[sc], cpv[sc], false);
                 if (MOM) {
-                    rk_store<TS>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tu, false);
-                    rk_store<TS>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tv, false);
-                    rk_store<TS>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tw, true);
+                    rk_store<TS>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tn[VU], cpv[VU], false);
+                    rk_store<TS>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tn[VV], cpv[VV], false);
+                    rk_store<TS>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tn[VW], cpv[VW], true);
                 }
             }
         }
@@ -353,11 +371,12 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int NC = 2 * (RM - 1);
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int kk = blockIdx.y * 4 + threadIdx.y;
+    const uint3 tile = xcd_tile<1>();  // XCD bands of 4-level blocks in z
+    const int i = tile.x * 64 + threadIdx.x;
+    const int kk = tile.y * 4 + threadIdx.y;
     if (i >= g.nx || kk >= g.nz) return;
     const int k = kk + g.gw;
-    const int j0 = blockIdx.z * jchunk;
+    const int j0 = tile.z * jchunk;
     const int j1 = min(j0 + jchunk, g.ny);
     const unsigned plane = (unsigned)g.plane;
     const unsigned nx = (unsigned)g.nx;
@@ -380,9 +399,12 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
         dxo[m] = wrapi(i + m - (RM - 2), g.nx) - i;
         dzo[m] = (m - (RM - 2)) * (int)plane;
     }
-    double bw[NSC > 0 ? NSC : 1][1];
+    double bw[NSC > 0 ? NSC : 1][1], brho[NSC > 0 ? NSC : 1];
 #pragma unroll
-    for (int sc = 0; sc < NSC; ++sc) bw[sc][0] = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
+    for (int sc = 0; sc < NSC; ++sc) {
+        bw[sc][0] = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
+        brho[sc] = bw[sc][0] * rh;
+    }
     TC ws[NSC > 0 ? NSC : 1][2 * RS];
     TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
     int c = j0 - 1;
@@ -407,6 +429,14 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     TC Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
     for (; c < j1; ++c) {
         const unsigned orow = lev + (unsigned)cw * nx;  // (i, c, k)
+        const bool upd = c >= j0;
+        // tendencies read before the flux arithmetic (see the z pass)
+        TS told[5] = {};
+        if (upd) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                if (v < NSC || (MOM && v >= VU)) told[v] = ld((const TS *)fl.t[v], orow);
+        }
         SFlux Fs[NSC > 0 ? NSC : 1];
         const TC V = MOM ? wsv[RM - 1] : (TC)ld(pv, orow);
 #pragma unroll
@@ -426,13 +456,14 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
             Fv = P::template mom_flux<MO>(wsv, Interp<MO, TC>::centred(q), rh);
             Fw = P::template mom_flux<MO>(wsw, centred_at<P, MO>(pv, oz), r0);  // v along z
         }
-        if (c >= j0) {
+        if (upd) {
 #pragma unroll
-            for (int sc = 0; sc < NSC; ++sc) *at((TS *)fl.t[sc], orow) -= P::scal_div(Fs[sc], Fs_prev[sc], ah, dyi);
+            for (int sc = 0; sc < NSC; ++sc)
+                *at((TS *)fl.t[sc], orow) = told[sc] - P::template scal_div<false>(Fs[sc], Fs_prev[sc], ah, dyi, brho[sc]);
             if (MOM) {
-                *at((TS *)fl.t[VU], orow) -= P::mom_div(Fu, Fu_prev, ah, dyi);
-                *at((TS *)fl.t[VV], orow) -= P::mom_div(Fv, Fv_prev, ah, dyi);
-                *at((TS *)fl.t[VW], orow) -= P::mom_div(Fw, Fw_prev, a0, dyi);
+                *at((TS *)fl.t[VU], orow) = told[VU] - P::mom_div(Fu, Fu_prev, ah, dyi);
+                *at((TS *)fl.t[VV], orow) = told[VV] - P::mom_div(Fv, Fv_prev, ah, dyi);
+                *at((TS *)fl.t[VW], orow) = told[VW] - P::mom_div(Fw, Fw_prev, a0, dyi);
             }
         }
 #pragma unroll
@@ -488,8 +519,9 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
     SFlux *sflux = reinterpret_cast<SFlux *>(smem_raw + fo);
     const size_t mo_ = ((fo + sizeof(SFlux) * (size_t)nx) + 15) & ~(size_t)15;
     TC *mflux = reinterpret_cast<TC *>(smem_raw + mo_);
-    const int j = blockIdx.x;
-    const int k = blockIdx.y + g.gw;
+    const uint3 tile = xcd_tile<0>();  // XCD bands of rows in y
+    const int j = tile.x;
+    const int k = tile.y + g.gw;
     const unsigned plane = (unsigned)g.plane;
     const unsigned row = ((unsigned)k * g.ny + j) * nx;
     const int tid = threadIdx.x;
@@ -525,7 +557,8 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
         __syncthreads();
         for (int c = tid; c < nx; c += B) {
             const int cm = c == 0 ? nx - 1 : c - 1;
-            sub_into((TS *)fl.t[sc], row + c, P::scal_div(sflux[c], sflux[cm], ah, dxi), fl.tzero != 0);
+            sub_into((TS *)fl.t[sc], row + c, P::template scal_div<false>(sflux[c], sflux[cm], ah, dxi, bw[0] * rh),
+                     fl.tzero != 0);
         }
         __syncthreads();
     }

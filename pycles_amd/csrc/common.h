@@ -59,6 +59,28 @@ __device__ __forceinline__ int wrapi(int i, int n) {
     return i < 0 ? i + n : (i >= n ? i - n : i);
 }
 
+// XCD-aware tile order.  The dispatcher deals workgroups round-robin over
+// the 8 XCDs by linear workgroup id, and each XCD has its own L2.  This maps
+// the linear id to a tile so that XCD x owns a contiguous band of grid
+// dimension BAND (0: x, 1: y) over the whole of the other dimensions; tiles
+// that read each other's cells along that dimension then share one L2.
+// A bijection of the grid for any dispatch order; the plain order when the
+// band dimension is not a multiple of 8.
+template <int BAND>
+__device__ __forceinline__ uint3 xcd_tile() {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned gb = BAND == 0 ? gx : gy;
+    if (gb % 8u != 0u) return make_uint3(blockIdx.x, blockIdx.y, blockIdx.z);
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned xcd = L % 8u, idx = L / 8u, band = gb / 8u;
+    if (BAND == 0) {
+        const unsigned t = idx / band;
+        return make_uint3(xcd * band + idx % band, t % gy, t / gy);
+    }
+    const unsigned t = idx / gx;
+    return make_uint3(idx % gx, xcd * band + t % band, t / band);
+}
+
 // ---------------------------------------------------------------------------
 // Interpolants.  The expression trees follow reference
 // Csrc/advection_interpolation.h (interp_4 :12-14, interp_6 :20-24,
