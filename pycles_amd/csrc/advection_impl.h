@@ -85,6 +85,12 @@ struct PolicyF64 {
     static __device__ __forceinline__ TC mom_div(TC Fc, TC Fp, double alpha, double dxi) {
         return alpha * (Fc - Fp) * dxi;
     }
+    // x pass: the flux kept in LDS, and its divergence from two kept fluxes
+    // and the face velocities
+    static __device__ __forceinline__ TC x_keep(SFlux F) { return F; }
+    static __device__ __forceinline__ TC x_scal_div(TC fc, TC fp, TC, TC, double alpha, double dxi, double) {
+        return alpha * (fc - fp) * dxi;
+    }
 };
 
 // -------------------------------------------------------------------------
@@ -156,6 +162,18 @@ struct PolicyF32 {
     }
     static __device__ __forceinline__ TC mom_div(TC Fc, TC Fp, double alpha, double dxi) {
         return (float)alpha * (Fc - Fp) * (float)dxi;
+    }
+    // x pass: only the FP32 part is kept; the base part is rebuilt from the
+    // face velocities, which is scal_div<false> exactly
+    static __device__ __forceinline__ TC x_keep(SFlux F) { return F.f; }
+    static __device__ __forceinline__ TC x_scal_div(TC fc, TC fp, TC Uc, TC Up, double alpha, double dxi,
+                                                    double brho) {
+        SF32 c, m;
+        c.f = fc;
+        c.b = (double)Uc;
+        m.f = fp;
+        m.b = (double)Up;
+        return scal_div<false>(c, m, alpha, dxi, brho);
     }
 };
 
@@ -498,27 +516,45 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
 }
 
 // -------------------------------------------------------------------------
-// x pass: one block per (j, k) row.  The row of every field (plus a periodic
-// halo of H cells) is staged in LDS; for each variable the block computes the
-// nx face fluxes into LDS, then updates the nx tendencies.
+// x pass: one block per (j, k) row, in three phases separated by barriers.
+//  1. stage in LDS: the rows of every field (plus a periodic halo of H
+//     cells), the advecting u centred along y and z for the v and w fluxes
+//     (gathered from the neighbouring rows and levels), and the tendencies
+//     the pass must read (w's, which thermodynamics wrote; all five unless
+//     the others are known to be zero).  Every global load is issued here.
+//  2. every face flux of every variable into LDS.
+//  3. the flux divergences, stored once per cell and variable.
+// LDS (TC words): rows NV*W | fluxes NF*nx | centred u 2*nx | old tendencies NT*nx.
+template <class P, int SO, int MO, int NSC, bool MOM>
+struct XLayout {
+    using TC = typename P::TC;
+    static constexpr int RS = Interp<SO, TC>::R;
+    static constexpr int RM = Interp<MO, TC>::R;
+    static constexpr int H = (MOM && RM > RS) ? RM : RS;
+    static constexpr int NV = NSC + (MOM ? 3 : 1);  // u row always staged (advecting velocity)
+    static constexpr int NF = NSC + (MOM ? 3 : 0);
+    // tendencies staged: w's when the others are zero on entry, else all
+    static __host__ __device__ int nt(bool tzero) { return tzero ? (MOM ? 1 : 0) : NF; }
+    static __host__ __device__ size_t words(int nx, bool tzero) {
+        return (size_t)NV * (nx + 2 * H) + (size_t)NF * nx + (MOM ? 2 * (size_t)nx : 0) + (size_t)nt(tzero) * nx;
+    }
+};
+
 template <class P, int SO, int MO, int NSC, bool MOM>
 __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
     using TS = typename P::TS;
     using TC = typename P::TC;
-    using SFlux = typename P::SFlux;
-    constexpr int RS = Interp<SO, TC>::R;
-    constexpr int RM = Interp<MO, TC>::R;
+    using L = XLayout<P, SO, MO, NSC, MOM>;
+    constexpr int RS = L::RS, RM = L::RM, H = L::H, NV = L::NV, NF = L::NF;
     constexpr int NC = 2 * (RM - 1);
-    constexpr int H = (MOM && RM > RS) ? RM : RS;
-    constexpr int NV = NSC + (MOM ? 3 : 1);  // u row always staged (advecting velocity)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nx = g.nx;
     const int W = nx + 2 * H;
+    const bool tzero = fl.tzero != 0;
     TC *rows = reinterpret_cast<TC *>(smem_raw);
-    const size_t fo = ((sizeof(TC) * (size_t)NV * W) + 15) & ~(size_t)15;
-    SFlux *sflux = reinterpret_cast<SFlux *>(smem_raw + fo);
-    const size_t mo_ = ((fo + sizeof(SFlux) * (size_t)nx) + 15) & ~(size_t)15;
-    TC *mflux = reinterpret_cast<TC *>(smem_raw + mo_);
+    TC *flux = rows + NV * W;          // flux[f * nx + c]: face c+1/2 of variable f
+    TC *ucen = flux + NF * nx;         // u centred along y (v flux) then along z (w flux)
+    TC *told = ucen + (MOM ? 2 * nx : 0);
     const uint3 tile = xcd_tile<0>();  // XCD bands of rows in y
     const int j = tile.x;
     const int k = tile.y + g.gw;
@@ -534,47 +570,95 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
         src[NSC + 1] = (const TS *)fl.q[VV];
         src[NSC + 2] = (const TS *)fl.q[VW];
     }
-    for (int v = 0; v < NV; ++v)
-        for (int t = tid; t < W; t += B) rows[v * W + t] = (TC)ld(src[v], row + wrapi(t - H, nx));
-    __syncthreads();
-    const double rh = pr.rho0_half[k];
-    const double r0 = pr.rho0[k];
-    const double ah = pr.alpha0_half[k];
-    const double a0 = pr.alpha0[k];
-    const double dxi = g.dxi[0];
-    const TC *urow = rows + NSC * W + H;  // urow[i] = u at cell i (wrapped halo)
+    // variable f of the flux rows -> field index (scalars, then u, v, w)
+    auto field = [](int f) { return f < NSC ? f : VU + (f - NSC); };
 
+    // phase 1, in batches whose loads are all issued before the LDS writes
+    // (a load written to LDS inside a plain loop waits once per element)
+    for (int t0 = tid; t0 < W; t0 += 4 * B) {
+        TC tmp[NV][4];
 #pragma unroll
-    for (int sc = 0; sc < NSC; ++sc) {
-        double bw[1] = {P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0};
-        const TC *srow = rows + sc * W + H;
-        for (int c = tid; c < nx; c += B) {
-            TC win[2 * RS];
+        for (int v = 0; v < NV; ++v)
 #pragma unroll
-            for (int m = 0; m < 2 * RS; ++m) win[m] = srow[c - (RS - 1) + m];
-            sflux[c] = P::template scal_flux<SO, false>(win, bw, urow[c], rh);
-        }
-        __syncthreads();
-        for (int c = tid; c < nx; c += B) {
-            const int cm = c == 0 ? nx - 1 : c - 1;
-            sub_into((TS *)fl.t[sc], row + c, P::template scal_div<false>(sflux[c], sflux[cm], ah, dxi, bw[0] * rh),
-                     fl.tzero != 0);
-        }
-        __syncthreads();
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + u * B;
+                tmp[v][u] = t < W ? (TC)ld(src[v], row + wrapi(t - H, nx)) : TC(0);
+            }
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (t0 + u * B < W) rows[v * W + t0 + u * B] = tmp[v][u];
     }
-    if (MOM) {
+    const int nt = L::nt(tzero);
+    const int f0 = NF - nt;  // the staged tendencies are those of the last nt variables
+    if (MOM || nt > 0) {
         const TS *pu = (const TS *)fl.q[VU];
-        // row offsets of u at the centred y and z stencils (v and w advected by u)
         int yo[NC], zo[NC];
 #pragma unroll
         for (int m = 0; m < NC; ++m) {
             yo[m] = (wrapi(j + m - (RM - 2), g.ny) - j) * nx;
             zo[m] = (m - (RM - 2)) * (int)plane;
         }
-        for (int a = 0; a < 3; ++a) {  // advected component u, v, w
-            const TC *prow = rows + (NSC + a) * W + H;
-            const double rho = (a == 2) ? r0 : rh;
-            for (int c = tid; c < nx; c += B) {
+        for (int c0 = tid; c0 < nx; c0 += 2 * B) {
+            TC uy[2] = {}, uz[2] = {}, tv[2][NF] = {};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int c = c0 + u * B;
+                if (c >= nx) continue;
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    if (f >= f0) tv[u][f] = (TC)ld((const TS *)fl.t[field(f)], row + c);
+                if (MOM) {
+                    unsigned oy[NC], oz[NC];
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) {
+                        oy[m] = row + c + yo[m];
+                        oz[m] = row + c + zo[m];
+                    }
+                    uy[u] = centred_at<P, MO>(pu, oy);
+                    uz[u] = centred_at<P, MO>(pu, oz);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int c = c0 + u * B;
+                if (c >= nx) continue;
+                if (MOM) {
+                    ucen[c] = uy[u];
+                    ucen[nx + c] = uz[u];
+                }
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    if (f >= f0) told[(f - f0) * nx + c] = tv[u][f];
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase 2
+    const double rh = pr.rho0_half[k];
+    const double r0 = pr.rho0[k];
+    const double ah = pr.alpha0_half[k];
+    const double a0 = pr.alpha0[k];
+    const double dxi = g.dxi[0];
+    const TC *urow = rows + NSC * W + H;  // urow[i] = u at cell i (wrapped halo)
+    double bw[NSC > 0 ? NSC : 1][1];
+#pragma unroll
+    for (int sc = 0; sc < NSC; ++sc) bw[sc][0] = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
+    for (int c = tid; c < nx; c += B) {
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) {
+            const TC *srow = rows + sc * W + H;
+            TC win[2 * RS];
+#pragma unroll
+            for (int m = 0; m < 2 * RS; ++m) win[m] = srow[c - (RS - 1) + m];
+            flux[sc * nx + c] = P::x_keep(P::template scal_flux<SO, false>(win, bw[sc], urow[c], rh));
+        }
+        if (MOM) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {  // advected component u, v, w
+                const TC *prow = rows + (NSC + a) * W + H;
                 TC win[2 * RM];
 #pragma unroll
                 for (int m = 0; m < 2 * RM; ++m) win[m] = prow[c - (RM - 1) + m];
@@ -585,41 +669,42 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
                     for (int m = 0; m < NC; ++m) q[m] = urow[c + m - (RM - 2)];
                     U = Interp<MO, TC>::centred(q);
                 } else {
-                    unsigned o[NC];
-#pragma unroll
-                    for (int m = 0; m < NC; ++m) o[m] = row + c + (a == 1 ? yo[m] : zo[m]);
-                    U = centred_at<P, MO>(pu, o);
+                    U = ucen[(a - 1) * nx + c];
                 }
-                mflux[c] = P::template mom_flux<MO>(win, U, rho);
+                flux[(NSC + a) * nx + c] = P::template mom_flux<MO>(win, U, a == 2 ? r0 : rh);
             }
-            __syncthreads();
-            const double al = (a == 2) ? a0 : ah;
-            for (int c = tid; c < nx; c += B) {
-                const int cm = c == 0 ? nx - 1 : c - 1;
-                sub_into((TS *)fl.t[VU + a], row + c, P::mom_div(mflux[c], mflux[cm], al, dxi),
-                         fl.tzero != 0 && a != 2);
+        }
+    }
+    __syncthreads();
+
+    // phase 3
+    for (int c = tid; c < nx; c += B) {
+        const int cm = c == 0 ? nx - 1 : c - 1;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            TS d;
+            if (f < NSC) {
+                d = P::x_scal_div(flux[f * nx + c], flux[f * nx + cm], urow[c], urow[cm], ah, dxi, bw[f][0] * rh);
+            } else {
+                d = P::mom_div(flux[f * nx + c], flux[f * nx + cm], f == NSC + 2 ? a0 : ah, dxi);
             }
-            __syncthreads();
+            // 0 - d when the tendency is zero on entry (same bits as adding to 0)
+            const int ti = f - (NF - nt);
+            const TS old = ti >= 0 ? (TS)told[ti * nx + c] : TS(0);
+            *at((TS *)fl.t[field(f)], row + c) = old - d;
         }
     }
 }
 
 template <class P, int SO, int MO, int NSC, bool MOM>
-size_t x_smem_bytes(int nx) {
-    using TC = typename P::TC;
-    constexpr int RS = Interp<SO, TC>::R;
-    constexpr int RM = Interp<MO, TC>::R;
-    constexpr int H = (MOM && RM > RS) ? RM : RS;
-    constexpr int NV = NSC + (MOM ? 3 : 1);
-    size_t fo = ((sizeof(TC) * (size_t)NV * (nx + 2 * H)) + 15) & ~(size_t)15;
-    size_t mo = ((fo + sizeof(typename P::SFlux) * (size_t)nx) + 15) & ~(size_t)15;
-    return mo + sizeof(TC) * nx + 16;
+size_t x_smem_bytes(int nx, bool tzero) {
+    return XLayout<P, SO, MO, NSC, MOM>::words(nx, tzero) * sizeof(typename P::TC);
 }
 
 // Launch the three passes in the reference's per-variable order (x, y, z).
 template <class P, int SO, int MO, int NSC, bool MOM>
 int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
-    const size_t smem = x_smem_bytes<P, SO, MO, NSC, MOM>(g.nx);
+    const size_t smem = x_smem_bytes<P, SO, MO, NSC, MOM>(g.nx, fl.tzero != 0);
     if (smem > 160 * 1024) {
         pycles_set_error("advection: row of %d cells does not fit in LDS", g.nx);
         return PYCLES_ERR_UNSUPPORTED;
