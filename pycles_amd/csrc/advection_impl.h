@@ -195,15 +195,15 @@ __device__ __forceinline__ typename P::TC centred_at(const typename P::TS *f, co
 // with the z boundary fill the cell feeds (set_bcs, prognostic_variables.h:
 // 205-270): sym mirrors the first/last gw levels; asym (w) zeroes the
 // boundary level and mirrors with a sign change.
-// cpv is the stage copy at o (read by the caller; unused in stage 0, which
-// writes it).
-template <typename TS>
+// RKM selects what the z pass stores: 0 the final tendencies, 1 RK3 stage 0
+// (writes the stage copy), 2 RK3 stage 1 or 2 (reads it; cpv is the copy).
+template <typename TS, int RKM>
 __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, unsigned colo, unsigned plane, int c,
                                          int gw, int nzg, TS vold, TS tn, TS cpv, bool asym) {
     TS *out = (TS *)fl.out[v];
     const TS dt = (TS)fl.dt;
     TS vn;
-    if (fl.rk == 0) {
+    if (RKM == 1) {
         *at((TS *)fl.cp[v], o) = vold;
         vn = rk_update<TS, 0>(vold, tn, TS(0), dt);
     } else if (fl.rk == 1) {
@@ -223,7 +223,13 @@ __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, un
     }
 }
 
-template <class P, int SO, int MO, int NSC, bool MOM>
+// Software pipelined: at the top of each face the loads of the next window
+// levels (clamped to the last ghost level) and of this face's centred w
+// stencils (used after the scalar fluxes) are issued, and this cell's
+// tendencies and stage copies together after the fluxes; all unconditional,
+// so one counted wait covers each group.  The first face (k0 - 1, flux only)
+// is peeled off the loop.
+template <class P, int SO, int MO, int NSC, bool MOM, int RKM>
 __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchunk) {
     using TS = typename P::TS;
     using TC = typename P::TC;
@@ -231,16 +237,17 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int NC = 2 * (RM - 1);
+    constexpr int NS = NSC > 0 ? NSC : 1;
     const uint3 tile = xcd_tile<1>();  // XCD bands of 4-row blocks in y
     const int i = tile.x * 64 + threadIdx.x;
     const int j = tile.y * 4 + threadIdx.y;
     if (i >= g.nx || j >= g.ny) return;  // no barriers in this kernel
     const int k0 = g.gw + tile.z * kchunk;
     const int k1 = min(k0 + kchunk, g.nz + g.gw);
+    const int ktop = g.nzg - 1;
     const unsigned plane = (unsigned)g.plane;
     const unsigned col = (unsigned)j * g.nx + i;
-    const TS *ps0 = (const TS *)fl.q[0];
-    const TS *ps1 = (const TS *)fl.q[1];
+    const TS *ps[2] = {(const TS *)fl.q[0], (const TS *)fl.q[1]};
     const TS *pu = (const TS *)fl.q[VU];
     const TS *pv = (const TS *)fl.q[VV];
     const TS *pw = (const TS *)fl.q[VW];
@@ -251,17 +258,20 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         cx[m] = (unsigned)j * g.nx + wrapi(i + m - (RM - 2), g.nx);
         cy[m] = (unsigned)wrapi(j + m - (RM - 2), g.ny) * g.nx + i;
     }
-    TC ws[NSC > 0 ? NSC : 1][2 * RS];
-    double bw[NSC > 0 ? NSC : 1][2 * RS];
+    auto base_at = [&](int sc, int lev) { return P::kBase ? (fl.base[sc] ? fl.base[sc][lev] : 0.0) : 0.0; };
+    // windows of face c: level c - (R-1) + m in slot m
+    TC ws[NS][2 * RS];
+    double bw[NS][2 * RS];
     TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
+    TC cwx[NC], cwy[NC];  // w along x and y at level c
     int c = k0 - 1;
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc)
 #pragma unroll
         for (int m = 0; m < 2 * RS; ++m) {
             const int lev = c - (RS - 1) + m;
-            ws[sc][m] = (TC)ld(sc ? ps1 : ps0, lev * plane + col);
-            bw[sc][m] = P::kBase ? (fl.base[sc] ? fl.base[sc][lev] : 0.0) : 0.0;
+            ws[sc][m] = (TC)ld(ps[sc], lev * plane + col);
+            bw[sc][m] = base_at(sc, lev);
         }
     if (MOM) {
 #pragma unroll
@@ -272,77 +282,120 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
             wsw[m] = (TC)ld(pw, o);
         }
     }
-    SFlux Fs_prev[NSC > 0 ? NSC : 1];
-    TC Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
-    const double dzi = g.dxi[2];
-
-    for (; c < k1; ++c) {
-        const unsigned lo = (unsigned)c * plane;
-        const unsigned o = lo + col;
-        const bool upd = c >= k0;
-        // This cell's tendencies (and stage copies) are read before the flux
-        // arithmetic: the compiler must assume the output arrays alias them,
-        // so a read placed next to its use could not move above the previous
-        // field's store, and each field would cost one memory round trip.
-        TS told[5] = {}, cpv[5] = {};
-        if (upd) {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                if (v < NSC || (MOM && v >= VU)) {
-                    told[v] = ld((const TS *)fl.t[v], o);
-                    if (fl.rk > 0) cpv[v] = ld((const TS *)fl.cp[v], o);
-                }
-            }
-        }
-        const double rho0c = pr.rho0[c];
-        const TC W = MOM ? wsw[RM - 1] : (TC)ld(pw, lo + col);
-        SFlux Fs[NSC > 0 ? NSC : 1];
-#pragma unroll
-        for (int sc = 0; sc < NSC; ++sc) Fs[sc] = P::template scal_flux<SO, true>(ws[sc], bw[sc], W, rho0c);
-        TC Fu = 0, Fv = 0, Fw = 0;
+    auto load_centred = [&](int cc) {
         if (MOM) {
-            unsigned ox[NC], oy[NC];
+            const unsigned lo = (unsigned)cc * plane;
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                ox[m] = lo + cx[m];
-                oy[m] = lo + cy[m];
+                cwx[m] = (TC)ld(pw, lo + cx[m]);
+                cwy[m] = (TC)ld(pw, lo + cy[m]);
             }
+        }
+    };
+    // next-face window loads, issued before the current face's arithmetic
+    TC nws[NS], nu = 0, nv = 0, nw = 0;
+    double nbw[NS];
+    auto issue_next = [&](int cc) {
+        const int ls = min(cc + 1 + RS, ktop);
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) {
+            nws[sc] = (TC)ld(ps[sc], ls * plane + col);
+            nbw[sc] = base_at(sc, ls);
+        }
+        if (MOM) {
+            const unsigned o = (unsigned)min(cc + 1 + RM, ktop) * plane + col;
+            nu = (TC)ld(pu, o);
+            nv = (TC)ld(pv, o);
+            nw = (TC)ld(pw, o);
+        }
+    };
+    auto advance = [&]() {
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) {
+#pragma unroll
+            for (int m = 0; m < 2 * RS - 1; ++m) {
+                ws[sc][m] = ws[sc][m + 1];
+                bw[sc][m] = bw[sc][m + 1];
+            }
+            ws[sc][2 * RS - 1] = nws[sc];
+            bw[sc][2 * RS - 1] = nbw[sc];
+        }
+        if (MOM) {
+#pragma unroll
+            for (int m = 0; m < 2 * RM - 1; ++m) {
+                wsu[m] = wsu[m + 1];
+                wsv[m] = wsv[m + 1];
+                wsw[m] = wsw[m + 1];
+            }
+            wsu[2 * RM - 1] = nu;
+            wsv[2 * RM - 1] = nv;
+            wsw[2 * RM - 1] = nw;
+        }
+    };
+    SFlux Fs[NS], Fs_prev[NS];
+    TC Fu = 0, Fv = 0, Fw = 0, Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
+    auto fluxes = [&](int cc, SFlux *F, TC &Fu_, TC &Fv_, TC &Fw_) {
+        const double rho0c = pr.rho0[cc];
+        const TC W = MOM ? wsw[RM - 1] : (TC)ld(pw, (unsigned)cc * plane + col);
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) F[sc] = P::template scal_flux<SO, true>(ws[sc], bw[sc], W, rho0c);
+        if (MOM) {
             // u advected by w: U = centred(w along x), rho0[k]
-            Fu = P::template mom_flux<MO>(wsu, centred_at<P, MO>(pw, ox), rho0c);
+            Fu_ = P::template mom_flux<MO>(wsu, Interp<MO, TC>::centred(cwx), rho0c);
             // v advected by w: U = centred(w along y), rho0[k]
-            Fv = P::template mom_flux<MO>(wsv, centred_at<P, MO>(pw, oy), rho0c);
+            Fv_ = P::template mom_flux<MO>(wsv, Interp<MO, TC>::centred(cwy), rho0c);
             // w advected by w: U = centred(w along z) from the window, rho0_half[k+1]
             TC q[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) q[m] = wsw[m + 1];
-            Fw = P::template mom_flux<MO>(wsw, Interp<MO, TC>::centred(q), pr.rho0_half[c + 1]);
+            Fw_ = P::template mom_flux<MO>(wsw, Interp<MO, TC>::centred(q), pr.rho0_half[cc + 1]);
         }
-        if (upd) {
-            const double ah = pr.alpha0_half[c];
-            // the final tendencies, with an uncontracted subtraction so the
-            // stored (separate RK3) and fused paths round identically
-            TS tn[5] = {};
+    };
+    const double dzi = g.dxi[2];
+    issue_next(c);
+    load_centred(c);
+    fluxes(c, Fs_prev, Fu_prev, Fv_prev, Fw_prev);
+    advance();
+    for (c = k0; c < k1; ++c) {
+        const unsigned o = (unsigned)c * plane + col;
+        issue_next(c);
+        load_centred(c);
+        fluxes(c, Fs, Fu, Fv, Fw);
+        // this cell's tendencies and stage copies, all read before any store
+        // (the compiler must assume the output arrays alias them); read after
+        // the flux arithmetic, which keeps them out of its register peak
+        TS told[5] = {}, cpv[5] = {};
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            if (v < NSC || (MOM && v >= VU)) {
+                told[v] = ld((const TS *)fl.t[v], o);
+                if (RKM == 2) cpv[v] = ld((const TS *)fl.cp[v], o);
+            }
+        }
+        const double ah = pr.alpha0_half[c];
+        // the final tendencies, with an uncontracted subtraction so the
+        // stored (separate RK3) and fused paths round identically
+        TS tn[5] = {};
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc)
+            tn[sc] = add_rn(told[sc], -P::template scal_div<true>(Fs[sc], Fs_prev[sc], ah, dzi, 1.0));
+        if (MOM) {
+            tn[VU] = add_rn(told[VU], -P::mom_div(Fu, Fu_prev, ah, dzi));
+            tn[VV] = add_rn(told[VV], -P::mom_div(Fv, Fv_prev, ah, dzi));
+            tn[VW] = add_rn(told[VW], -P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
+        }
+        if (RKM == 0) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                if (v < NSC || (MOM && v >= VU)) *at((TS *)fl.t[v], o) = tn[v];
+        } else {
 #pragma unroll
             for (int sc = 0; sc < NSC; ++sc)
-                tn[sc] = add_rn(told[sc], -P::template scal_div<true>(Fs[sc], Fs_prev[sc], ah, dzi, 1.0));
+                rk_store<TS, RKM>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn[sc], cpv[sc], false);
             if (MOM) {
-                tn[VU] = add_rn(told[VU], -P::mom_div(Fu, Fu_prev, ah, dzi));
-                tn[VV] = add_rn(told[VV], -P::mom_div(Fv, Fv_prev, ah, dzi));
-                tn[VW] = add_rn(told[VW], -P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
-            }
-            if (fl.rk < 0) {
-#pragma unroll
-                for (int v = 0; v < 5; ++v)
-                    if (v < NSC || (MOM && v >= VU)) *at((TS *)fl.t[v], o) = tn[v];
-            } else {
-#pragma unroll
-                for (int sc = 0; sc < NSC; ++sc)
-                    rk_store<TS>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn[sc], cpv[sc], false);
-                if (MOM) {
-                    rk_store<TS>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tn[VU], cpv[VU], false);
-                    rk_store<TS>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tn[VV], cpv[VV], false);
-                    rk_store<TS>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tn[VW], cpv[VW], true);
-                }
+                rk_store<TS, RKM>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tn[VU], cpv[VU], false);
+                rk_store<TS, RKM>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tn[VV], cpv[VV], false);
+                rk_store<TS, RKM>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tn[VW], cpv[VW], true);
             }
         }
 #pragma unroll
@@ -350,37 +403,16 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         Fu_prev = Fu;
         Fv_prev = Fv;
         Fw_prev = Fw;
-        if (c + 1 < k1) {
-#pragma unroll
-            for (int sc = 0; sc < NSC; ++sc) {
-#pragma unroll
-                for (int m = 0; m < 2 * RS - 1; ++m) {
-                    ws[sc][m] = ws[sc][m + 1];
-                    bw[sc][m] = bw[sc][m + 1];
-                }
-                const int lev = c + 1 + RS;
-                ws[sc][2 * RS - 1] = (TC)ld(sc ? ps1 : ps0, lev * plane + col);
-                bw[sc][2 * RS - 1] = P::kBase ? (fl.base[sc] ? fl.base[sc][lev] : 0.0) : 0.0;
-            }
-            if (MOM) {
-#pragma unroll
-                for (int m = 0; m < 2 * RM - 1; ++m) {
-                    wsu[m] = wsu[m + 1];
-                    wsv[m] = wsv[m + 1];
-                    wsw[m] = wsw[m + 1];
-                }
-                const unsigned o = (unsigned)(c + 1 + RM) * plane + col;
-                wsu[2 * RM - 1] = (TC)ld(pu, o);
-                wsv[2 * RM - 1] = (TC)ld(pv, o);
-                wsw[2 * RM - 1] = (TC)ld(pw, o);
-            }
-        }
+        advance();
     }
 }
 
 // -------------------------------------------------------------------------
 // y pass: one thread per (i, k) with k interior, marching faces c = j0-1 .. j1-1
 // (periodic).  Block (64, 4); grid (ceil(nx/64), ceil(nz/4), nchunks).
+// Pipelined like the z pass: the next row of every window and this face's
+// centred v stencils are loaded at the top of each face, the tendencies
+// together after the fluxes; the first face is peeled off the loop.
 template <class P, int SO, int MO, int NSC, bool MOM>
 __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchunk) {
     using TS = typename P::TS;
@@ -389,6 +421,7 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     constexpr int RS = Interp<SO, TC>::R;
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int NC = 2 * (RM - 1);
+    constexpr int NS = NSC > 0 ? NSC : 1;
     const uint3 tile = xcd_tile<1>();  // XCD bands of 4-level blocks in z
     const int i = tile.x * 64 + threadIdx.x;
     const int kk = tile.y * 4 + threadIdx.y;
@@ -399,8 +432,7 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     const unsigned plane = (unsigned)g.plane;
     const unsigned nx = (unsigned)g.nx;
     const unsigned lev = (unsigned)k * plane + i;  // offset of (i, 0, k)
-    const TS *ps0 = (const TS *)fl.q[0];
-    const TS *ps1 = (const TS *)fl.q[1];
+    const TS *ps[2] = {(const TS *)fl.q[0], (const TS *)fl.q[1]};
     const TS *pu = (const TS *)fl.q[VU];
     const TS *pv = (const TS *)fl.q[VV];
     const TS *pw = (const TS *)fl.q[VW];
@@ -409,28 +441,28 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     const double ah = pr.alpha0_half[k];
     const double a0 = pr.alpha0[k];
     const double dyi = g.dxi[1];
-    // offsets (relative to row start) of the centred stencils of v along x and z
-    int dxo[NC];
-    int dzo[NC];
+    // offsets (relative to the row) of the centred stencils of v along x and z
+    int dxo[NC], dzo[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
         dxo[m] = wrapi(i + m - (RM - 2), g.nx) - i;
         dzo[m] = (m - (RM - 2)) * (int)plane;
     }
-    double bw[NSC > 0 ? NSC : 1][1], brho[NSC > 0 ? NSC : 1];
+    double bw[NS][1], brho[NS];
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc) {
         bw[sc][0] = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
         brho[sc] = bw[sc][0] * rh;
     }
-    TC ws[NSC > 0 ? NSC : 1][2 * RS];
+    TC ws[NS][2 * RS];
     TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
+    TC cvx[NC], cvz[NC];  // v along x and z at row c
     int c = j0 - 1;
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc)
 #pragma unroll
         for (int m = 0; m < 2 * RS; ++m)
-            ws[sc][m] = (TC)ld(sc ? ps1 : ps0, lev + (unsigned)wrapi(c - (RS - 1) + m, g.ny) * nx);
+            ws[sc][m] = (TC)ld(ps[sc], lev + (unsigned)wrapi(c - (RS - 1) + m, g.ny) * nx);
     if (MOM) {
 #pragma unroll
         for (int m = 0; m < 2 * RM; ++m) {
@@ -440,78 +472,103 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
             wsw[m] = (TC)ld(pw, o);
         }
     }
-    // next rows to load (wrapped) for the scalar and momentum windows
-    int jn_s = wrapi(c + RS, g.ny), jn_m = wrapi(c + RM, g.ny);
+    // rows (wrapped) of the entries that enter the windows at face c + 1,
+    // c + 1 + R, and of the current face
+    int jn_s = wrapi(c + 1 + RS, g.ny), jn_m = wrapi(c + 1 + RM, g.ny);
     int cw = c < 0 ? c + g.ny : c;
-    SFlux Fs_prev[NSC > 0 ? NSC : 1];
-    TC Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
-    for (; c < j1; ++c) {
-        const unsigned orow = lev + (unsigned)cw * nx;  // (i, c, k)
-        const bool upd = c >= j0;
-        // tendencies read before the flux arithmetic (see the z pass)
-        TS told[5] = {};
-        if (upd) {
+    TC nws[NS], nu = 0, nv = 0, nw = 0;
+    auto issue_next = [&]() {
 #pragma unroll
-            for (int v = 0; v < 5; ++v)
-                if (v < NSC || (MOM && v >= VU)) told[v] = ld((const TS *)fl.t[v], orow);
-        }
-        SFlux Fs[NSC > 0 ? NSC : 1];
-        const TC V = MOM ? wsv[RM - 1] : (TC)ld(pv, orow);
-#pragma unroll
-        for (int sc = 0; sc < NSC; ++sc) Fs[sc] = P::template scal_flux<SO, false>(ws[sc], bw[sc], V, rh);
-        TC Fu = 0, Fv = 0, Fw = 0;
+        for (int sc = 0; sc < NSC; ++sc) nws[sc] = (TC)ld(ps[sc], lev + (unsigned)jn_s * nx);
         if (MOM) {
-            unsigned ox[NC], oz[NC];
+            const unsigned o = lev + (unsigned)jn_m * nx;
+            nu = (TC)ld(pu, o);
+            nv = (TC)ld(pv, o);
+            nw = (TC)ld(pw, o);
+        }
+    };
+    auto load_centred = [&](unsigned orow) {
+        if (MOM) {
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
-                ox[m] = orow + dxo[m];
-                oz[m] = orow + dzo[m];
+                cvx[m] = (TC)ld(pv, orow + dxo[m]);
+                cvz[m] = (TC)ld(pv, orow + dzo[m]);
             }
-            Fu = P::template mom_flux<MO>(wsu, centred_at<P, MO>(pv, ox), rh);  // v along x
+        }
+    };
+    auto advance = [&]() {
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) {
+#pragma unroll
+            for (int m = 0; m < 2 * RS - 1; ++m) ws[sc][m] = ws[sc][m + 1];
+            ws[sc][2 * RS - 1] = nws[sc];
+        }
+        if (MOM) {
+#pragma unroll
+            for (int m = 0; m < 2 * RM - 1; ++m) {
+                wsu[m] = wsu[m + 1];
+                wsv[m] = wsv[m + 1];
+                wsw[m] = wsw[m + 1];
+            }
+            wsu[2 * RM - 1] = nu;
+            wsv[2 * RM - 1] = nv;
+            wsw[2 * RM - 1] = nw;
+        }
+        cw = cw + 1 == g.ny ? 0 : cw + 1;
+        jn_s = jn_s + 1 == g.ny ? 0 : jn_s + 1;
+        jn_m = jn_m + 1 == g.ny ? 0 : jn_m + 1;
+    };
+    SFlux Fs[NS], Fs_prev[NS];
+    TC Fu = 0, Fv = 0, Fw = 0, Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
+    auto fluxes = [&](unsigned orow, SFlux *F, TC &Fu_, TC &Fv_, TC &Fw_) {
+        const TC V = MOM ? wsv[RM - 1] : (TC)ld(pv, orow);
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) F[sc] = P::template scal_flux<SO, false>(ws[sc], bw[sc], V, rh);
+        // the scalar fluxes cover the latency of the centred stencil loads
+        __builtin_amdgcn_sched_barrier(0);
+        if (MOM) {
+            Fu_ = P::template mom_flux<MO>(wsu, Interp<MO, TC>::centred(cvx), rh);  // v along x
             TC q[NC];
 #pragma unroll
             for (int m = 0; m < NC; ++m) q[m] = wsv[m + 1];  // v along y
-            Fv = P::template mom_flux<MO>(wsv, Interp<MO, TC>::centred(q), rh);
-            Fw = P::template mom_flux<MO>(wsw, centred_at<P, MO>(pv, oz), r0);  // v along z
+            Fv_ = P::template mom_flux<MO>(wsv, Interp<MO, TC>::centred(q), rh);
+            Fw_ = P::template mom_flux<MO>(wsw, Interp<MO, TC>::centred(cvz), r0);  // v along z
         }
-        if (upd) {
+    };
+    {
+        const unsigned orow = lev + (unsigned)cw * nx;
+        issue_next();
+        load_centred(orow);
+        fluxes(orow, Fs_prev, Fu_prev, Fv_prev, Fw_prev);
+        advance();
+    }
+    for (c = j0; c < j1; ++c) {
+        const unsigned orow = lev + (unsigned)cw * nx;  // (i, c, k)
+        issue_next();
+        load_centred(orow);
+        // keep the loads above the flux arithmetic (the scheduler would sink
+        // them next to their first use)
+        __builtin_amdgcn_sched_barrier(0);
+        fluxes(orow, Fs, Fu, Fv, Fw);
+        // tendencies, all read before any store (see the z pass)
+        TS told[5] = {};
 #pragma unroll
-            for (int sc = 0; sc < NSC; ++sc)
-                *at((TS *)fl.t[sc], orow) = told[sc] - P::template scal_div<false>(Fs[sc], Fs_prev[sc], ah, dyi, brho[sc]);
-            if (MOM) {
-                *at((TS *)fl.t[VU], orow) = told[VU] - P::mom_div(Fu, Fu_prev, ah, dyi);
-                *at((TS *)fl.t[VV], orow) = told[VV] - P::mom_div(Fv, Fv_prev, ah, dyi);
-                *at((TS *)fl.t[VW], orow) = told[VW] - P::mom_div(Fw, Fw_prev, a0, dyi);
-            }
+        for (int v = 0; v < 5; ++v)
+            if (v < NSC || (MOM && v >= VU)) told[v] = ld((const TS *)fl.t[v], orow);
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc)
+            *at((TS *)fl.t[sc], orow) = told[sc] - P::template scal_div<false>(Fs[sc], Fs_prev[sc], ah, dyi, brho[sc]);
+        if (MOM) {
+            *at((TS *)fl.t[VU], orow) = told[VU] - P::mom_div(Fu, Fu_prev, ah, dyi);
+            *at((TS *)fl.t[VV], orow) = told[VV] - P::mom_div(Fv, Fv_prev, ah, dyi);
+            *at((TS *)fl.t[VW], orow) = told[VW] - P::mom_div(Fw, Fw_prev, a0, dyi);
         }
 #pragma unroll
         for (int sc = 0; sc < NSC; ++sc) Fs_prev[sc] = Fs[sc];
         Fu_prev = Fu;
         Fv_prev = Fv;
         Fw_prev = Fw;
-        cw = cw + 1 == g.ny ? 0 : cw + 1;
-        jn_s = jn_s + 1 == g.ny ? 0 : jn_s + 1;
-        jn_m = jn_m + 1 == g.ny ? 0 : jn_m + 1;
-        if (c + 1 < j1) {
-#pragma unroll
-            for (int sc = 0; sc < NSC; ++sc) {
-#pragma unroll
-                for (int m = 0; m < 2 * RS - 1; ++m) ws[sc][m] = ws[sc][m + 1];
-                ws[sc][2 * RS - 1] = (TC)ld(sc ? ps1 : ps0, lev + (unsigned)jn_s * nx);
-            }
-            if (MOM) {
-#pragma unroll
-                for (int m = 0; m < 2 * RM - 1; ++m) {
-                    wsu[m] = wsu[m + 1];
-                    wsv[m] = wsv[m + 1];
-                    wsw[m] = wsw[m + 1];
-                }
-                const unsigned o = lev + (unsigned)jn_m * nx;
-                wsu[2 * RM - 1] = (TC)ld(pu, o);
-                wsv[2 * RM - 1] = (TC)ld(pv, o);
-                wsw[2 * RM - 1] = (TC)ld(pw, o);
-            }
-        }
+        advance();
     }
 }
 
@@ -737,8 +794,13 @@ int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
     while (zcols * zch < 2048 && g.nz / (zch * 2) >= 32) zch *= 2;
     const int kchunk = (g.nz + zch - 1) / zch;
     if (fl.passes & 4) {
-        hipLaunchKernelGGL((adv_z<P, SO, MO, NSC, MOM>), dim3((g.nx + 63) / 64, (g.ny + 3) / 4, zch),
-                           dim3(64, 4), 0, st, g, pr, fl, kchunk);
+        const dim3 zg((g.nx + 63) / 64, (g.ny + 3) / 4, zch);
+        if (fl.rk < 0)
+            hipLaunchKernelGGL((adv_z<P, SO, MO, NSC, MOM, 0>), zg, dim3(64, 4), 0, st, g, pr, fl, kchunk);
+        else if (fl.rk == 0)
+            hipLaunchKernelGGL((adv_z<P, SO, MO, NSC, MOM, 1>), zg, dim3(64, 4), 0, st, g, pr, fl, kchunk);
+        else
+            hipLaunchKernelGGL((adv_z<P, SO, MO, NSC, MOM, 2>), zg, dim3(64, 4), 0, st, g, pr, fl, kchunk);
         PYCLES_CHECK_LAUNCH();
     }
     return PYCLES_OK;
