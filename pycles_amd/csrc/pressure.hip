@@ -36,8 +36,8 @@ struct pycles_pressure_plan {
     int nxh;
     int gx;          // kx columns per y-FFT block
     void *spec;      // [nz][ny][nxh] complex, field precision
-    double *xs;      // Thomas forward-sweep values, complex FP64
-    double *cs;      // Thomas forward-sweep coefficients, FP64
+    void *xs;        // Thomas forward-sweep values, complex, storage type
+    void *cs;        // Thomas forward-sweep coefficients, storage type
     void *twx;       // exp(-2 pi i m / nx), m < nx/2, interleaved (field precision)
     void *twy;
     double *kx2, *ky2, *rho_h, *dd, *a, *c;
@@ -274,8 +274,12 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
 // One thread per (kx, ky) column: TDMA of SparseSolvers.pyx:24-43 on the real
 // and imaginary parts, with the diagonal of compute_diagonal
 // (PressureFFTParallel.pyx:129-148); the (0,0) mode is set to zero (:196-200).
+// The recurrence runs in FP64; its scratch (coefficients and forward-swept
+// values, written by the forward sweep and read back by the back
+// substitution) is stored in the storage type TS: FP64 for the FP64 form,
+// FP32 for the FP32 form, whose right-hand side and solution are FP32 already.
 template <typename TS>
-__global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double *xs, double *cs,
+__global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs, TS *cs,
                                                      const double *kx2, const double *ky2, const double *rho_h,
                                                      const double *dd, const double *a, const double *c, int nxh) {
     constexpr int PF = 8;  // levels loaded ahead of the recurrence
@@ -314,9 +318,9 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double
                     xr = ((double)z.x - a[k] * xr) * m;
                     xi = ((double)z.y - a[k] * xi) * m;
                 }
-                cs[o] = sc;
-                xs[2 * o] = xr;
-                xs[2 * o + 1] = xi;
+                cs[o] = (TS)sc;
+                xs[2 * o] = (TS)xr;
+                xs[2 * o + 1] = (TS)xi;
             }
         }
     }
@@ -328,9 +332,9 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double
         const int k = nz - 2 - u;
         if (k >= 0) {
             const long long o = (long long)k * ncol + col;
-            sq[u] = cs[o];
-            rq[u] = xs[2 * o];
-            iq[u] = xs[2 * o + 1];
+            sq[u] = (double)cs[o];
+            rq[u] = (double)xs[2 * o];
+            iq[u] = (double)xs[2 * o + 1];
         }
     }
     for (int k0 = nz - 2; k0 >= 0; k0 -= PF) {
@@ -341,9 +345,9 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, double
                 const double s_ = sq[u], r_ = rq[u], i_ = iq[u];
                 if (k - PF >= 0) {
                     const long long on = (long long)(k - PF) * ncol + col;
-                    sq[u] = cs[on];
-                    rq[u] = xs[2 * on];
-                    iq[u] = xs[2 * on + 1];
+                    sq[u] = (double)cs[on];
+                    rq[u] = (double)xs[2 * on];
+                    iq[u] = (double)xs[2 * on + 1];
                 }
                 xr = r_ - s_ * xr;
                 xi = i_ - s_ * xi;
@@ -580,8 +584,8 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
         }                                                   \
     } while (0)
     ALLOC(P->spec, nspec * cs);
-    ALLOC(P->xs, nspec * 16);
-    ALLOC(P->cs, nspec * 8);
+    ALLOC(P->xs, nspec * cs);
+    ALLOC(P->cs, nspec * (cs / 2));
     ALLOC(P->twx, (size_t)g.nx * cs);
     ALLOC(P->twy, (size_t)g.ny * cs);
     ALLOC(P->kx2, P->nxh * 8);
@@ -653,7 +657,8 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
     PYCLES_CHECK_LAUNCH();
     const long long ncol = (long long)g.ny * P->nxh;
     hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
-                       (C2<TS> *)P->spec, P->xs, P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c, P->nxh);
+                       (C2<TS> *)P->spec, (TS *)P->xs, (TS *)P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c,
+                       P->nxh);
     PYCLES_CHECK_LAUNCH();
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 1);
