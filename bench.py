@@ -50,6 +50,14 @@ METRIC = "Mcell-updates/s per RK3 step: FP32+learned vs plain FP64 at <=1e-5 rel
 #           their tendencies, reads (stages 1, 2) or writes (stage 0) the stage
 #           copies, writes the five new values                   20 x 4
 KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, "adv_z_rk3": 80}
+# Algorithmic flops per interior cell of the advection passes: WENO5 scalar and
+# momentum advection is ~2265 flop per cell per stage over the three directions
+# (2 scalar and 9 momentum face fluxes, interpolation and divergence), split
+# evenly over the directional passes; the fused RK3 stage adds 4 flop per
+# variable.  VALU peak: FP32 vector with packed FMA (the kernels issue no
+# packed FP32, so half of it is their ceiling).
+KERNEL_FLOPS = {"adv_x": 755, "adv_y": 755, "adv_z": 755, "adv_z_rk3": 775}
+VALU_PEAK_TFLOPS = 157.3
 KERNEL_NAMES = {"thermo": "thermo_sa_f32_kernel<true>", "adv_x": "adv::adv_x<PolicyF32,5,5,2,true>",
                 "adv_y": "adv::adv_y<PolicyF32,5,5,2,true>", "adv_z": "adv::adv_z<PolicyF32,5,5,2,true>",
                 "rk3": "rk3_kernel<float,...>", "adv_z_rk3": "adv::adv_z<PolicyF32,5,5,2,true> (RK3 fused)"}
@@ -268,11 +276,18 @@ def main():
     torch.cuda.empty_cache()
 
     kernels = per_kernel(phL, args.steps)
-    traffic = measured_traffic()
+    traffic = measured_traffic() if args.config == "moist512" else {}
     for k, v in kernels.items():
         ach = v["bytes_per_cell"] * ncell / (v["ms_per_launch"] * 1e-3) / 1e9
         v["achieved_GBs"] = round(ach, 1)
         v["hbm_frac"] = round(ach / HBM_PEAK_GBS, 4)
+        if k in KERNEL_FLOPS:
+            tf = KERNEL_FLOPS[k] * ncell / (v["ms_per_launch"] * 1e-3) / 1e12
+            v["flop_per_cell"] = KERNEL_FLOPS[k]
+            v["achieved_TFLOPs"] = round(tf, 2)
+            v["valu_frac"] = round(tf / VALU_PEAK_TFLOPS, 4)
+        if k in traffic:
+            v["traffic_bytes_per_launch"] = traffic[k]["bytes_per_launch"]
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_launch"])
     kd = kernels[dom]
     tr = traffic.get(dom)

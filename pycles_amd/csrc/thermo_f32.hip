@@ -18,8 +18,11 @@ struct MlpSmem {
     alignas(16) float red[4][mlp::RED_FLOATS];
 };
 
+// Waves per SIMD: the plain kernel is bound by memory and dependency
+// latency, and runs faster at 6 waves even with a few spills; the network's
+// registers keep the learned kernel at 4.
 template <bool MLP>
-__global__ __launch_bounds__(256, 4) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
+__global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
     extern __shared__ double dyn[];
     double *tab = dyn;
     float *tabf = (float *)(tab + a.ntab);
