@@ -106,7 +106,11 @@ int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void 
  * each cell as soon as its tendency is final and writes the new value, and
  * the ghost levels it feeds, to out_* (arrays distinct from the inputs; the
  * caller swaps them in).  copy_* are the stage copies (written at rk_step 0,
- * read at 1 and 2).  The tendency arrays keep the x and y contributions.
+ * read at 1 and 2).  At rk_step 0 they may be NULL: no copy is written, and
+ * the caller keeps that stage's input unchanged and passes it as the copy of
+ * stages 1 and 2; at rk_step 2 out_* may be that same array (each cell's copy
+ * is read before its output is written).  The tendency arrays keep the x and
+ * y contributions.
  * Same bits as pycles_advection then pycles_rk3_stage.  Needs the momentum. */
 int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
                          void *const tend_vel[3], int nscal, const void *const *scal,

@@ -75,18 +75,29 @@ class MomentumAdvection:
                         PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
 
 
-def advect_rk3(SA: ScalarAdvection, MA: MomentumAdvection, TS, Gr, RS, PV, tend_zero=False, passes=0):
-    """advect_fused followed by TS.update in one sweep (pycles_advection_rk3):
-    the new values go to PV's second array, which is then made current."""
+def advect_rk3(SA: ScalarAdvection, MA: MomentumAdvection, TS, Gr, RS, PV, tend_zero=False, passes=0,
+               arrays=None):
+    """advect_fused followed by TS.update in one sweep (pycles_advection_rk3).
+    arrays = (out, copy) from PV.rk_arrays(TS.rk_step) (three rotating value
+    arrays; no stage copy written); None: the new values go to PV's second
+    array and the copies to TS's copy array.  The caller makes the output
+    array current afterwards."""
     names = _scalar_names(PV)
     ref = D.device_ref(RS)
     vn = PV.velocity_names_directional
     tv = [PV.tptr(vn[d]) for d in range(3)]
+    if arrays is None:
+        outs = [PV.ptr_next(vn[d]) for d in range(3)] + [PV.ptr_next(n) for n in names]
+        cps = [TS.copy_ptr(PV, vn[d]) for d in range(3)] + [TS.copy_ptr(PV, n) for n in names]
+    else:
+        out, cp = arrays
+        outs = [out[PV.get_nv(vn[d])].data_ptr() for d in range(3)] + [out[PV.get_nv(n)].data_ptr() for n in names]
+        cps = None if cp is None else ([cp[PV.get_nv(vn[d])].data_ptr() for d in range(3)] +
+                                       [cp[PV.get_nv(n)].data_ptr() for n in names])
     _capi.advection_rk3(SA.dims, ref.c, _vel(PV), tv, [PV.ptr(n) for n in names],
                         [PV.tptr(n) for n in names], _base(PV, names), SA.order, MA.order,
                         PV.precision, D.stream(), (_capi.ADV_TEND_ZERO if tend_zero else 0) | passes,
-                        [PV.ptr_next(vn[d]) for d in range(3)], [PV.ptr_next(n) for n in names],
-                        [TS.copy_ptr(PV, vn[d]) for d in range(3)], [TS.copy_ptr(PV, n) for n in names],
+                        outs[:3], outs[3:], None if cps is None else cps[:3], None if cps is None else cps[3:],
                         TS.rk_step, TS.dt)
 
 

@@ -97,6 +97,26 @@ class PrognosticVariables(_State):
         """Make the array the fused stage wrote the current one."""
         self.values, self.values_next = self.values_next, self.values
 
+    def rk_arrays(self, rk_step):
+        """Output and copy value arrays of a fused RK3 stage
+        (pycles_advection_rk3).  Three arrays rotate: the stage-0 input stays
+        unchanged through the step and is the copy of stages 1 and 2, so
+        stage 0 writes no copy, and stage 2 writes its output over it (each
+        cell's copy is read before its output is written).  Returns (out,
+        copy), copy None at stage 0; make `out` current with set_values."""
+        if getattr(self, "_vbufs", None) is None or not any(b is self.values for b in self._vbufs):
+            self._vbufs = [self.values, torch.zeros_like(self.values), torch.zeros_like(self.values)]
+        if rk_step == 0:
+            self._rk_base = self.values
+            self._rk_spare = [b for b in self._vbufs if b is not self.values]
+            return self._rk_spare[0], None
+        if rk_step == 1:
+            return self._rk_spare[1], self._rk_base
+        return self._rk_base, self._rk_base
+
+    def set_values(self, arr):
+        self.values = arr
+
     def tend(self, name) -> torch.Tensor:
         return self.tendencies[self.name_index[name]]
 

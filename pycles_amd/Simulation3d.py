@@ -117,10 +117,12 @@ class Simulation3d:
                           ("adv_z_rk3", _capi.ADV_PASS_Z))
             else:
                 passes = (("advection_rk3", 0),)
+            arrays = self.PV.rk_arrays(rk_step)
             for name, bit in passes:
                 self._mark(name)
-                advect_rk3(self.SA, self.MA, self.TS, self.Gr, self.Ref, self.PV, tend_zero=fast, passes=bit)
-            self.PV.swap_values()
+                advect_rk3(self.SA, self.MA, self.TS, self.Gr, self.Ref, self.PV, tend_zero=fast, passes=bit,
+                           arrays=arrays)
+            self.PV.set_values(arrays[0])
             self.TS.advance()
         elif self.fuse_advection and self.timers is not None:
             # the same three kernels as the fused call, with an event between

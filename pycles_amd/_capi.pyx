@@ -253,19 +253,21 @@ def advection_rk3(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase
     cdef void *osc[2]
     cdef void *csc[2]
     cdef int ns = len(scal)
+    # copy_vel / copy_scal None: no stage copies (allowed at rk_step 0)
+    cdef bint has_copy = copy_vel is not None
     for i in range(3):
         v3[i] = _p(vel[i])
         t3[i] = _p(tend_vel[i])
         o3[i] = _p(out_vel[i])
-        c3[i] = _p(copy_vel[i])
+        c3[i] = _p(copy_vel[i]) if has_copy else NULL
     for i in range(ns):
         sc[i] = _p(scal[i])
         ts[i] = _p(tend_scal[i])
         osc[i] = _p(out_scal[i])
-        csc[i] = _p(copy_scal[i])
+        csc[i] = _p(copy_scal[i]) if has_copy else NULL
     _check(pycles_advection_rk3(&dims.d, &ref.r, v3, t3, ns, sc, ts, &base.b if base is not None else NULL,
-                                scalar_order, momentum_order, precision, flags, o3, osc, c3, csc, rk_step, dt,
-                                _p(stream)))
+                                scalar_order, momentum_order, precision, flags, o3, osc,
+                                c3 if has_copy else NULL, csc if has_copy else NULL, rk_step, dt, _p(stream)))
 
 
 def thermo_sa(Dims dims, Ref ref, Lookup lt, s, qt, wt, T, qv, ql, alpha, buoyancy, theta_rho,

@@ -104,8 +104,11 @@ extern "C" int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *r
         pycles_set_error("advection_rk3: rk_step %d not in [0, 2]", rk_step);
         return PYCLES_ERR_ARG;
     }
-    if (!tend_vel || !out_vel || !copy_vel || nscal < 0 || nscal > 2 || (nscal > 0 && (!out_scal || !copy_scal))) {
-        pycles_set_error("advection_rk3: needs the momentum and an output and copy array for every field");
+    const bool need_copy = rk_step > 0;  // at stage 0 the copies may be NULL (none written)
+    if (!tend_vel || !out_vel || (need_copy && !copy_vel) || nscal < 0 || nscal > 2 ||
+        (nscal > 0 && (!out_scal || (need_copy && !copy_scal)))) {
+        pycles_set_error("advection_rk3: needs the momentum and an output (and, at rk_step 1 and 2, a copy) "
+                         "array for every field");
         return PYCLES_ERR_ARG;
     }
     if (flags & (PYCLES_ADV_PASS_X | PYCLES_ADV_PASS_Y | PYCLES_ADV_PASS_Z)) {
@@ -121,17 +124,17 @@ extern "C" int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *r
     rk.dt = dt;
     for (int s = 0; s < nscal; ++s) {
         rk.out[s] = out_scal[s];
-        rk.cp[s] = copy_scal[s];
+        rk.cp[s] = copy_scal ? copy_scal[s] : nullptr;
     }
     for (int d = 0; d < 3; ++d) {
         rk.out[adv::VU + d] = out_vel[d];
-        rk.cp[adv::VU + d] = copy_vel[d];
+        rk.cp[adv::VU + d] = copy_vel ? copy_vel[d] : nullptr;
     }
     for (int v = 0; v < 5; ++v) {
         if (v < nscal || v >= adv::VU) {
             const void *in = v < nscal ? scal[v] : vel[v - adv::VU];
-            if (!rk.out[v] || !rk.cp[v] || rk.out[v] == in) {
-                pycles_set_error("advection_rk3: output %d is NULL or the input array itself", v);
+            if (!rk.out[v] || (need_copy && !rk.cp[v]) || rk.out[v] == in || rk.cp[v] == in) {
+                pycles_set_error("advection_rk3: output or copy %d is NULL or the input array itself", v);
                 return PYCLES_ERR_ARG;
             }
         }

@@ -196,7 +196,9 @@ __device__ __forceinline__ typename P::TC centred_at(const typename P::TS *f, co
 // 205-270): sym mirrors the first/last gw levels; asym (w) zeroes the
 // boundary level and mirrors with a sign change.
 // RKM selects what the z pass stores: 0 the final tendencies, 1 RK3 stage 0
-// (writes the stage copy), 2 RK3 stage 1 or 2 (reads it; cpv is the copy).
+// (writes the stage copy, unless its pointer is NULL), 2 RK3 stage 1 or 2
+// (reads it; cpv is the copy, read before the output is written, so the
+// output may be the copy array itself).
 template <typename TS, int RKM>
 __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, unsigned colo, unsigned plane, int c,
                                          int gw, int nzg, TS vold, TS tn, TS cpv, bool asym) {
@@ -204,7 +206,7 @@ __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, un
     const TS dt = (TS)fl.dt;
     TS vn;
     if (RKM == 1) {
-        *at((TS *)fl.cp[v], o) = vold;
+        if (fl.cp[v]) *at((TS *)fl.cp[v], o) = vold;  // NULL: the caller keeps the input as the copy
         vn = rk_update<TS, 0>(vold, tn, TS(0), dt);
     } else if (fl.rk == 1) {
         vn = rk_update<TS, 1>(vold, tn, cpv, dt);
