@@ -68,7 +68,10 @@ class TimeStepping:
                           self.cfl_dev.data_ptr(), PV.precision, D.stream())
         self._cfl_ready = False
         self.cfl_max = float(self.cfl_dev.item())
-        if self.cfl_max < 0.0:
+        # the reference stops on a negative maximum; a NaN velocity makes the
+        # device maximum NaN (its bits exceed every finite value's), and that
+        # stops the run too instead of setting a NaN time step
+        if not self.cfl_max >= 0.0:
             raise RuntimeError(f"CFL_MAX = {self.cfl_max} killing simulation!")
 
     def cfl_time_step(self):

@@ -98,3 +98,21 @@ def test_initial_velocities_are_divergence_free(moist):
     div = sim.divergence()[gw:-gw, gw:-gw, gw:-gw]
     scale = np.abs(Gr.from_device_layout(fields["u"])).max() / d.dx[0]
     assert np.abs(div).max() < 1e-12 * scale
+
+
+def test_nan_cfl_stops_the_run():
+    """A NaN CFL maximum (NaN velocities) raises instead of giving a NaN dt."""
+    import math
+    import torch
+    from pycles_amd.TimeStepping import TimeStepping
+    ts = TimeStepping.__new__(TimeStepping)
+    ts.dt, ts.dt_max, ts.cfl_limit = 1.0, 10.0, 0.7
+    ts._cfl_ready = True  # the value is already in the device slot
+    for bad in (math.nan, -1.0):
+        ts.cfl_dev = torch.tensor([bad], dtype=torch.float64)
+        with pytest.raises(RuntimeError, match="killing simulation"):
+            ts.compute_cfl_max(None, None)
+        ts._cfl_ready = True
+    ts.cfl_dev = torch.tensor([0.35], dtype=torch.float64)
+    ts.compute_cfl_max(None, None)
+    assert ts.cfl_time_step() == 2.0
