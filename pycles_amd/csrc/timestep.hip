@@ -240,16 +240,24 @@ __global__ __launch_bounds__(256) void level_partial_kernel(Geo g, const T *f, d
     const long long lo = sp * per;
     const long long hi = lo + per < n ? lo + per : n;
     const T *row = f + (long long)k * n;
-    double acc = 0.0;
-    for (long long c = lo + threadIdx.x; c < hi; c += blockDim.x) acc += (double)row[c];
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    // four independent sums in flight per thread (fixed order: deterministic)
+    const int B = blockDim.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    long long c = lo + threadIdx.x;
+    for (; c + 3 * B < hi; c += 4 * B) {
+        a0 += (double)row[c];
+        a1 += (double)row[c + B];
+        a2 += (double)row[c + 2 * B];
+        a3 += (double)row[c + 3 * B];
     }
-    if (threadIdx.x == 0) partial[(long long)k * MEAN_SPLITS + sp] = red[0];
+    for (; c < hi; c += B) a0 += (double)row[c];
+    double acc = (a0 + a1) + (a2 + a3);
+    // wavefront sum by lane shuffles, then the block's four wavefronts
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[(long long)k * MEAN_SPLITS + sp] = (red[0] + red[1]) + (red[2] + red[3]);
     (void)base;
 }
 
