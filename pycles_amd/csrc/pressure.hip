@@ -194,34 +194,57 @@ __global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const
     const double a0 = alpha0[k], a0m = alpha0[k - 1];
     const double mk = mean[k], mkm = mean[k - 1];
     const double dxi0 = 1.0 / g.dx[0], dxi1 = 1.0 / g.dx[1], dxi2 = 1.0 / g.dx[2];
-    for (int t = threadIdx.x; t < (g.nx << (logS + 1)); t += blockDim.x) {
-        const int i = t & (g.nx - 1);
-        const int rb = t >> logx;
-        const int j = j0 + rb;
-        const int jm = j == 0 ? g.ny - 1 : j - 1;
-        const int im1 = i == 0 ? g.nx - 1 : i - 1;
-        const long long o = gidx(g, i, j, k);
-        TS d;
-        if (sizeof(TS) == 8) {
-            double dv = 0.0;
-            dv += ((double)u[o] / ah - (double)u[gidx(g, im1, j, k)] / ah) * dxi0;
-            dv += ((double)v[o] / ah - (double)v[gidx(g, i, jm, k)] / ah) * dxi1;
-            const double wk = (double)w[o] - mk;
-            const double wkm = (double)w[o - g.plane] - mkm;
-            dv += ((wk) / a0 - (wkm) / a0m) * dxi2;
-            d = (TS)dv;
-        } else {
-            // FP32 path: rho0*u in FP32 except the level means and profiles
-            const float rh = (float)(1.0 / ah);
-            float dv = ((float)u[o] - (float)u[gidx(g, im1, j, k)]) * (float)(dxi0)*rh;
-            dv += ((float)v[o] - (float)v[gidx(g, i, jm, k)]) * (float)(dxi1)*rh;
-            const float wk = (float)((double)w[o] - mk);
-            const float wkm = (float)((double)w[o - g.plane] - mkm);
-            dv += (wk * (float)(1.0 / a0) - wkm * (float)(1.0 / a0m)) * (float)dxi2;
-            d = (TS)dv;
+    // four cells per thread at a time: all their loads are issued before the
+    // arithmetic and the stores (a store in between would hold back the
+    // next cell's loads, which the compiler must assume it aliases)
+    const int B = blockDim.x, nt = g.nx << (logS + 1);
+    for (int t0 = threadIdx.x; t0 < nt; t0 += 4 * B) {
+        TS uc[4], um[4], vc[4], vm[4], wc[4], wm[4];
+        long long oo[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = min(t0 + q * B, nt - 1);  // a clamped duplicate past the end: loaded, not stored
+            const int i = t & (g.nx - 1);
+            const int j = j0 + (t >> logx);
+            const int jm = j == 0 ? g.ny - 1 : j - 1;
+            const int im1 = i == 0 ? g.nx - 1 : i - 1;
+            const long long o = gidx(g, i, j, k);
+            oo[q] = o;
+            uc[q] = u[o];
+            um[q] = u[gidx(g, im1, j, k)];
+            vc[q] = v[o];
+            vm[q] = v[gidx(g, i, jm, k)];
+            wc[q] = w[o];
+            wm[q] = w[o - g.plane];
         }
-        if (div_out) div_out[o] = d;
-        bufr[2 * (i * S + (rb >> 1)) + (rb & 1)] = d;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int t = t0 + q * B;
+            if (t >= nt) continue;
+            const int i = t & (g.nx - 1);
+            const int rb = t >> logx;
+            TS d;
+            if (sizeof(TS) == 8) {
+                double dv = 0.0;
+                dv += ((double)uc[q] / ah - (double)um[q] / ah) * dxi0;
+                dv += ((double)vc[q] / ah - (double)vm[q] / ah) * dxi1;
+                const double wk = (double)wc[q] - mk;
+                const double wkm = (double)wm[q] - mkm;
+                dv += ((wk) / a0 - (wkm) / a0m) * dxi2;
+                d = (TS)dv;
+            } else {
+                // FP32 path: rho0*u in FP32 except the level means and profiles
+                const float rh = (float)(1.0 / ah);
+                float dv = ((float)uc[q] - (float)um[q]) * (float)(dxi0)*rh;
+                dv += ((float)vc[q] - (float)vm[q]) * (float)(dxi1)*rh;
+                const float wk = (float)((double)wc[q] - mk);
+                const float wkm = (float)((double)wm[q] - mkm);
+                dv += (wk * (float)(1.0 / a0) - wkm * (float)(1.0 / a0m)) * (float)dxi2;
+                d = (TS)dv;
+            }
+            if (div_out) div_out[oo[q]] = d;
+            bufr[2 * (i * S + (rb >> 1)) + (rb & 1)] = d;
+        }
     }
     __syncthreads();
     const int b = threadIdx.x & (S - 1), t = threadIdx.x >> logS;
@@ -249,9 +272,19 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
     const int kk = blockIdx.y;
     const int ncol = min(gx, nxh - kx0);
     C2<TS> *lev = spec + (long long)kk * g.ny * nxh;
-    for (int q = threadIdx.x; q < g.ny * gx; q += blockDim.x) {
-        const int y = q >> loggx, b = q & (gx - 1);
-        buf[q] = b < ncol ? lev[(long long)y * nxh + kx0 + b] : C2<TS>{TS(0), TS(0)};
+    // staged four elements per thread at a time, loads before LDS writes
+    const int B = blockDim.x, nq = g.ny * gx;
+    for (int q0 = threadIdx.x; q0 < nq; q0 += 4 * B) {
+        C2<TS> z[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * B;
+            const int y = q >> loggx, b = q & (gx - 1);
+            z[u] = (q < nq && b < ncol) ? lev[(long long)y * nxh + kx0 + b] : C2<TS>{TS(0), TS(0)};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q0 + u * B < nq) buf[q0 + u * B] = z[u];
     }
     __syncthreads();
     const int b = threadIdx.x & (gx - 1), t = threadIdx.x >> loggx;
@@ -370,22 +403,36 @@ __global__ __launch_bounds__(512) void ifftx_kernel(Geo g, const C2<TS> *spec, T
     const TS *bufr = reinterpret_cast<const TS *>(smem_raw);
     const int kk = blockIdx.y;
     const int j0 = blockIdx.x << (logS + 1);
-    for (int q = threadIdx.x; q < (g.nx << logS); q += blockDim.x) {
-        const int m = q & (g.nx - 1);
-        const int pb = q >> logx;
-        const C2<TS> *ra = spec + ((long long)kk * g.ny + j0 + 2 * pb) * nxh;
-        const C2<TS> *rb = ra + nxh;
-        C2<TS> xa, xb;
-        if (m < nxh) {
-            xa = ra[m];
-            xb = rb[m];
-        } else {
-            xa = ra[g.nx - m];
-            xb = rb[g.nx - m];
-            xa.y = -xa.y;
-            xb.y = -xb.y;
+    // staged four elements per thread at a time: their loads are all issued
+    // before the LDS writes (one memory round trip per four, not per one)
+    const int B = blockDim.x, nq = g.nx << logS;
+    for (int q0 = threadIdx.x; q0 < nq; q0 += 4 * B) {
+        C2<TS> xa[4], xb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * B;
+            if (q < nq) {
+                const int m = q & (g.nx - 1);
+                const int pb = q >> logx;
+                const C2<TS> *ra = spec + ((long long)kk * g.ny + j0 + 2 * pb) * nxh;
+                const int mm = m < nxh ? m : g.nx - m;
+                xa[u] = ra[mm];
+                xb[u] = ra[nxh + mm];
+            }
         }
-        buf[m * S + pb] = C2<TS>{xa.x - xb.y, xa.y + xb.x};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u * B;
+            if (q < nq) {
+                const int m = q & (g.nx - 1);
+                const int pb = q >> logx;
+                if (m >= nxh) {  // Hermitian extension X[N-m] = conj X[m]
+                    xa[u].y = -xa[u].y;
+                    xb[u].y = -xb[u].y;
+                }
+                buf[m * S + pb] = C2<TS>{xa[u].x - xb[u].y, xa[u].y + xb[u].x};
+            }
+        }
     }
     __syncthreads();
     const int b = threadIdx.x & (S - 1), t = threadIdx.x >> logS;
