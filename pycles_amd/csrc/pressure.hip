@@ -468,33 +468,49 @@ __global__ __launch_bounds__(256) void correct_kernel(Geo g, TS *u, TS *v, TS *w
         const long long ce = (long long)j * g.nx + ip;
         const long long cn = (long long)jp * g.nx + i;
         TS pc = p[(long long)k0 * g.plane + col];
-#pragma unroll 4
-        for (int k = k0; k < k1; ++k) {
-            const long long lo = (long long)k * g.plane;
-            const long long o = lo + col;
-            const TS pe = p[lo + ce];
-            const TS pn = p[lo + cn];
-            const TS pt = p[o + g.plane];
-            TS un, vn, wn;
-            if (sizeof(TS) == 8) {
-                un = u[o] - (pe - pc) * g.dxi[0];
-                vn = v[o] - (pn - pc) * g.dxi[1];
-                const double wm = (double)w[o] - mean[k];
-                wn = (TS)(wm - ((double)pt - (double)pc) * g.dxi[2]);
-            } else {
-                un = u[o] - (pe - pc) * (TS)g.dxi[0];
-                vn = v[o] - (pn - pc) * (TS)g.dxi[1];
-                wn = (TS)((double)w[o] - mean[k]) - (pt - pc) * (TS)g.dxi[2];
+        // four levels at a time: all their loads before their stores (a store
+        // would hold back the next level's loads, which it may alias)
+        for (int kb = k0; kb < k1; kb += 4) {
+            TS uq[4], vq[4], wq[4], peq[4], pnq[4], ptq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = min(kb + q, k1 - 1);  // a clamped duplicate past the end: loaded, not stored
+                const long long lo = (long long)k * g.plane;
+                const long long o = lo + col;
+                peq[q] = p[lo + ce];
+                pnq[q] = p[lo + cn];
+                ptq[q] = p[o + g.plane];
+                uq[q] = u[o];
+                vq[q] = v[o];
+                wq[q] = w[o];
             }
-            u[o] = un;
-            v[o] = vn;
-            w[o] = wn;
-            if (cfl) {
-                const double c = dt * (fabs((double)un) * g.dxi[0] + fabs((double)vn) * g.dxi[1] +
-                                       fabs((double)wn) * g.dxi[2]);
-                cm = fmax(cm, c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = kb + q;
+                if (k >= k1) break;
+                const long long o = (long long)k * g.plane + col;
+                const TS pe = peq[q], pn = pnq[q], pt = ptq[q];
+                TS un, vn, wn;
+                if (sizeof(TS) == 8) {
+                    un = uq[q] - (pe - pc) * g.dxi[0];
+                    vn = vq[q] - (pn - pc) * g.dxi[1];
+                    const double wm = (double)wq[q] - mean[k];
+                    wn = (TS)(wm - ((double)pt - (double)pc) * g.dxi[2]);
+                } else {
+                    un = uq[q] - (pe - pc) * (TS)g.dxi[0];
+                    vn = vq[q] - (pn - pc) * (TS)g.dxi[1];
+                    wn = (TS)((double)wq[q] - mean[k]) - (pt - pc) * (TS)g.dxi[2];
+                }
+                u[o] = un;
+                v[o] = vn;
+                w[o] = wn;
+                if (cfl) {
+                    const double c = dt * (fabs((double)un) * g.dxi[0] + fabs((double)vn) * g.dxi[1] +
+                                           fabs((double)wn) * g.dxi[2]);
+                    cm = fmax(cm, c);
+                }
+                pc = pt;
             }
-            pc = pt;
         }
         // z ghost levels from the corrected boundary levels this thread wrote
         // (chunks hold at least 2 gw levels)
