@@ -336,21 +336,37 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ P, 
     const int t = threadIdx.x;
     if (t == 0) {
         w->nm = *nmp;
-        w->b3 = P[OB3];
+        float b3s = P[OB3];
+        for (int u = 0; u < NH; ++u) b3s += bf16_round(P[OW3 + u]);
+        w->b3s = b3s;
     }
-    for (int q = t; q < NH; q += blockDim.x) {
-        w->W1b[q] = make_float4(TANH_C * bf16_round(P[OW1 + 3 * q]), TANH_C * bf16_round(P[OW1 + 3 * q + 1]),
-                                TANH_C * bf16_round(P[OW1 + 3 * q + 2]), TANH_C * P[OB1 + q]);
-        w->b2c[q] = TANH_C * P[OB2 + q];
-        w->W3[q] = bf16_round(P[OW3 + q]);
+    for (int q = t; q < 2 * 64; q += blockDim.x) {
+        const int lane = q % 64, nb = q / 64;
+        const int u = 32 * nb + lane % 32;
+        bf16x8 a = {};
+        if (lane < 32) {
+            const float cb = TANH_C * P[OB1 + u];
+            const __bf16 bh = (__bf16)cb;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) a[2 * v] = a[2 * v + 1] = (__bf16)P[OW1 + 3 * u + v];
+            a[6] = bh;
+            a[7] = (__bf16)(cb - (float)bh);
+        }
+        w->A1[nb][lane] = a;
+    }
+    for (int q = t; q < 2 * 2 * 16; q += blockDim.x) {
+        const int r = q % 16, h = (q / 16) % 2, nb = q / 32;
+        const int u = 32 * nb + d_unit(h, r);
+        w->C2[nb][h][r] = TANH_C * P[OB2 + u];
+        w->W3n[nb][h][r] = -2.0f * bf16_round(P[OW3 + u]);
     }
     for (int q = t; q < 2 * 4 * 64; q += blockDim.x) {
-        const int lane = q % 64, kk = (q / 64) % 4, nt = q / 256;
+        const int lane = q % 64, qq = (q / 64) % 4, nb = q / 256;
+        const int row = 32 * nb + lane % 32, h = lane / 32;
         bf16x8 b;
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
-            b[e] = (__bf16)P[OW2 + (32 * nt + lane % 32) * NH + 16 * kk + 8 * (lane / 32) + e];
-        w->W2B[q] = b;
+        for (int e = 0; e < 8; ++e) b[e] = (__bf16)(TANH_C * P[OW2 + row * NH + k_unit(qq, h, e)]);
+        w->A2[nb][qq][lane] = b;
     }
 }
 
@@ -360,7 +376,6 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ P, 
 __global__ __launch_bounds__(256) void infer_kernel(const Weights *__restrict__ wg, const double *__restrict__ data,
                                                     long long n, float *out) {
     __shared__ Weights w;
-    __shared__ __align__(16) float red[4][RED_FLOATS];
     stage(wg, &w, threadIdx.x, blockDim.x);
     __syncthreads();
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -368,7 +383,7 @@ __global__ __launch_bounds__(256) void infer_kernel(const Weights *__restrict__ 
     const double *d = data + 4 * ic;
     float x0, x1, x2;
     normalise(&w, d[0], (float)d[1], (float)d[2], &x0, &x1, &x2);
-    const float T = guess_wave(&w, red[threadIdx.x >> 6], x0, x1, x2);
+    const float T = guess_wave(&w, x0, x1, x2);
     if (i < n) out[i] = T;
 }
 

@@ -83,14 +83,16 @@ __device__ __forceinline__ double resid64(double s_red, double p0, double qt, do
 }
 
 // The FP64 secant for the rare rejected cell, kept out of line so its
-// registers do not weigh on the FP32 path.
-__device__ __noinline__ void fallback64(const thermo::Table &lt64, double p0, double s, double qt, int max_iter,
-                                        float *T, float *qv, float *ql) {
+// registers do not weigh on the FP32 path.  Arguments and results pass by
+// value: a pointer to the caller's T, qv, ql would keep those in scratch
+// memory for the whole level loop, with a full memory wait at every reload.
+struct TQ {
+    float T, qv, ql;
+};
+__device__ __noinline__ TQ fallback64(thermo::Table lt64, double p0, double s, double qt, int max_iter) {
     double T64, qv64, ql64, qi64;
     thermo::eos_sa(lt64, p0, s, qt, &T64, &qv64, &ql64, &qi64, max_iter);
-    *T = (float)T64;
-    *qv = (float)qv64;
-    *ql = (float)ql64;
+    return TQ{(float)T64, (float)qv64, (float)ql64};
 }
 
 // State after the unsaturated test of eos_c (thermodynamics_sa.h:23-33).
@@ -211,7 +213,10 @@ __device__ __forceinline__ int eos_sa_f32_finish(const TableF &lt, const thermo:
     }
     it += secant_f32(lt, st, p0f, qt, max_iter, T, qv, ql);
     if (!passes_check(lt64, p0, s_red64, qt64, *T)) {
-        fallback64(lt64, p0, s, qt64, max_iter, T, qv, ql);
+        const TQ r = fallback64(lt64, p0, s, qt64, max_iter);
+        *T = r.T;
+        *qv = r.qv;
+        *ql = r.ql;
         *fallback = 1;
     }
     return it;

@@ -11,22 +11,31 @@
 // kernel carries none of the network's registers or LDS).  The guess is
 // evaluated by whole wavefronts (one row of 64 columns at one level) when any
 // of their cells is saturated; every lane stays in the loop for that reason.
-// Dynamic LDS: the pv* table in FP64 and FP32, then the chunk's exner values.
+// Dynamic LDS: the pv* table in FP64, the chunk's p0 and base profiles (FP64),
+// the table in FP32, then the chunk's exner and alpha0 values.  Every per-level
+// value comes from LDS: a vector load of one would make each level wait for
+// all the loads and stores before it (the memory counter completes in order).
+// WTO: the w tendency is overwritten, not added to (no read of it).
 struct NoSmem {};
 struct MlpSmem {
     mlp::Weights w;
-    alignas(16) float red[4][mlp::RED_FLOATS];
 };
 
 // Waves per SIMD: the plain kernel is bound by memory and dependency
-// latency, and runs faster at 6 waves even with a few spills; the network's
-// registers keep the learned kernel at 4.
-template <bool MLP>
-__global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
+// latency; at 5 waves it keeps its registers (at 6 its spill reloads, which
+// wait for every outstanding load and store, cost more than the extra wave
+// gains); the network's registers keep the learned kernel at 4.
+template <bool MLP, bool WTO>
+__global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
     extern __shared__ double dyn[];
+    const int nl = a.kchunk + 2;  // levels of a chunk with its two neighbours
     double *tab = dyn;
-    float *tabf = (float *)(tab + a.ntab);
+    double *p0l = tab + a.ntab;
+    double *sbl = p0l + nl;
+    double *qbl = sbl + nl;
+    float *tabf = (float *)(qbl + nl);
     float *exner = tabf + a.ntab;
+    float *a0l = exner + nl;
     __shared__ unsigned long long cnt[4];
     __shared__ typename std::conditional<MLP, MlpSmem, NoSmem>::type msm_store;
     MlpSmem *msm = reinterpret_cast<MlpSmem *>(&msm_store);
@@ -37,7 +46,14 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
         tab[t] = a.tab[t];
         tabf[t] = (float)a.tab[t];
     }
-    for (int k = ch.kw + tid; k < ch.ke; k += nthr) exner[k - ch.kw] = (float)thermo::exner_c(a.p0[k]);
+    for (int k = ch.kw + tid; k < ch.ke; k += nthr) {
+        const int l = k - ch.kw;
+        exner[l] = (float)thermo::exner_c(a.p0[k]);
+        p0l[l] = a.p0[k];
+        sbl[l] = a.sbase ? a.sbase[k] : 0.0;
+        qbl[l] = a.qbase ? a.qbase[k] : 0.0;
+        a0l[l] = (float)a.alpha0_half[k];
+    }
     if (tid < 4) cnt[tid] = 0;
     if (MLP) mlp::stage(a.mlp, &msm->w, tid, nthr);
     __syncthreads();
@@ -55,22 +71,27 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
     const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
     const float dzi = (float)(1.0 / g.dx[2]);
     float Tw[3] = {0, 0, 0}, qtw[3] = {0, 0, 0}, qvw[3] = {0, 0, 0}, thw[3] = {0, 0, 0}, bprev = 0.0f;
-    // s and qt of the next level are loaded one iteration ahead
-    float s_nx = active ? s[(long long)ch.kw * g.plane + col] : 0.0f;
-    float q_nx = active ? qtp[(long long)ch.kw * g.plane + col] : 0.0f;
+    // s and qt of the next level are loaded one iteration ahead, by every
+    // lane and at every level (the level clamped to the chunk; lanes outside
+    // the domain read column 0): a load under a branch would make the wait
+    // for it count from the branch's other path
+    float s_nx = s[(long long)ch.kw * g.plane + col];
+    float q_nx = qtp[(long long)ch.kw * g.plane + col];
     for (int k = ch.kw; k < ch.ke; ++k) {
+        const int l = k - ch.kw;
         const bool own = active && k >= ch.k0 && k < ch.k1;
         const bool own_m = active && k - 1 >= ch.k0 && k - 1 < ch.k1;
         const long long o = (long long)k * g.plane + col;
-        const double p0 = a.p0[k];
+        const double p0 = p0l[l];
         const float p0f = (float)p0;
-        const double sb = a.sbase ? a.sbase[k] : 0.0;
-        const double qb = a.qbase ? a.qbase[k] : 0.0;
+        const double sb = sbl[l];
+        const double qb = qbl[l];
         float sd = s_nx;
         float qd = q_nx;
-        if (active && k + 1 < ch.ke) {
-            s_nx = s[o + g.plane];
-            q_nx = qtp[o + g.plane];
+        {
+            const long long on = (long long)min(k + 1, ch.ke - 1) * g.plane + col;
+            s_nx = s[on];
+            q_nx = qtp[on];
         }
         double qt64 = qb + (double)qd;
         if (a.clip && !(qt64 >= 1e-11)) {
@@ -89,7 +110,7 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
             if (__ballot(sat)) {
                 float x0, x1, x2;
                 mlp::normalise(&msm->w, s64, qt, p0f, &x0, &x1, &x2);
-                T_guess = mlp::guess_wave(&msm->w, msm->red[threadIdx.y], x0, x1, x2);
+                T_guess = mlp::guess_wave(&msm->w, x0, x1, x2);
             }
         }
         float T = st.T_1, qv = qt, ql = 0.0f;
@@ -104,10 +125,10 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
             ctr[3] += fb;
         }
         const float al = (float)PC_RD * T / p0f * (1.0f - qt + (float)PC_EPS_VI * qv);
-        const float ex = exner[k - ch.kw];
+        const float ex = exner[l];
         float b = 0.0f;
         if (k < g.nzg - 1) {
-            const float a0 = (float)a.alpha0_half[k];
+            const float a0 = a0l[l];
             b = (float)PC_G * (al - a0) / a0;
         }
         float thr = 0.0f;
@@ -132,7 +153,7 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
             }
         }
         if (own_m && k >= 2 && k - 1 < g.nzg - 3) {
-            const float w0 = a.wt_over ? 0.0f : wt[o - g.plane];
+            const float w0 = WTO ? 0.0f : wt[o - g.plane];
             wt[o - g.plane] = w0 + 0.5f * (bprev + b);
         }
         bprev = b;
@@ -143,7 +164,7 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
                 const float Tk = Tw[1];
                 const float Lv = thermo32::latent_heat(Tk);
                 const float pvs = thermo32::lookup(ltf, Tk);
-                const float p0m = (float)a.p0[m];
+                const float p0m = (float)p0l[l - 1];
                 const float rsl = (float)PC_EPS_V * pvs / (p0m - pvs);
                 const float cpm = (1.0f - qtw[1]) * (float)PC_CPD + qtw[1] * (float)PC_CPV;
                 const float gamma_w = (float)(PC_G / PC_CPD) * (1.0f / (1.0f - qtw[1])) *
@@ -163,16 +184,26 @@ __global__ __launch_bounds__(256, MLP ? 4 : 6) void thermo_sa_f32_kernel(Geo g, 
     flush_counters(cnt, ctr, a.counters, tid);
 }
 
-// Dynamic LDS of the FP32 kernel: table (FP64 + FP32) and the chunk's exner.
+// Dynamic LDS of the FP32 kernel: table (FP64 + FP32), three FP64 and two
+// FP32 values per level of the chunk.
 static size_t f32_smem_bytes(const SAArgs &a) {
-    return (size_t)a.ntab * (sizeof(double) + sizeof(float)) + (size_t)(a.kchunk + 2) * sizeof(float);
+    return (size_t)a.ntab * (sizeof(double) + sizeof(float)) +
+           (size_t)(a.kchunk + 2) * (3 * sizeof(double) + 2 * sizeof(float));
+}
+
+template <bool MLP>
+static void launch_f32(const Geo &g, const SAArgs &a, dim3 grd, dim3 blk, hipStream_t st) {
+    if (a.wt_over)
+        hipLaunchKernelGGL((thermo_sa_f32_kernel<MLP, true>), grd, blk, f32_smem_bytes(a), st, g, a);
+    else
+        hipLaunchKernelGGL((thermo_sa_f32_kernel<MLP, false>), grd, blk, f32_smem_bytes(a), st, g, a);
 }
 
 int thermo_sa_f32_launch(const Geo &g, const SAArgs &a, dim3 grd, dim3 blk, hipStream_t st) {
     if (a.mlp)
-        hipLaunchKernelGGL(thermo_sa_f32_kernel<true>, grd, blk, f32_smem_bytes(a), st, g, a);
+        launch_f32<true>(g, a, grd, blk, st);
     else
-        hipLaunchKernelGGL(thermo_sa_f32_kernel<false>, grd, blk, f32_smem_bytes(a), st, g, a);
+        launch_f32<false>(g, a, grd, blk, st);
     PYCLES_CHECK_LAUNCH();
     return PYCLES_OK;
 }
