@@ -6,7 +6,10 @@
 // (grid z) for parallelism: a thread owns levels [k0, k1), starts one level
 // early to fill its windows and reads one level past k1; it writes only the
 // outputs of the levels it owns.  The exner function of every level is
-// computed once per block into LDS.
+// computed once per block into LDS, with the other per-level values (p0,
+// alpha0): a vector load of one inside the level loop would make each level
+// wait for all the loads and stores before it (the memory counter completes
+// in order).  s and qt of the next level are loaded one level ahead.
 //
 //   ThermodynamicsSA.update       ThermodynamicsSA.pyx:176-213
 //   clip_qt                       thermodynamics_sa.h:213-220
@@ -20,16 +23,28 @@
 #include "thermo_common.h"
 
 // ---------------------------------------------------------------------------
-// FP64: the reference's arithmetic.
+// FP64: the reference's arithmetic.  Dynamic LDS: the pv* table, then the
+// chunk's exner, p0 and alpha0 values.  WTO: the w tendency is overwritten,
+// not added to (no read of it).
+template <bool WTO>
 __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
-    __shared__ double tab[LOOKUP_MAX];
-    __shared__ double exner[LEVELS_MAX];
+    extern __shared__ double dyn64[];
+    const int nl = a.kchunk + 2;  // levels of a chunk with its two neighbours
+    double *tab = dyn64;
+    double *exner = tab + a.ntab;
+    double *p0l = exner + nl;
+    double *a0l = p0l + nl;
     __shared__ unsigned long long cnt[4];
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int nthr = blockDim.x * blockDim.y;
     const Chunk ch = chunk_of(g, a.kchunk);
     for (int t = tid; t < a.ntab; t += nthr) tab[t] = a.tab[t];
-    for (int k = ch.kw + tid; k < ch.ke; k += nthr) exner[k] = thermo::exner_c(a.p0[k]);
+    for (int k = ch.kw + tid; k < ch.ke; k += nthr) {
+        const int l = k - ch.kw;
+        exner[l] = thermo::exner_c(a.p0[k]);
+        p0l[l] = a.p0[k];
+        a0l[l] = a.alpha0_half[k];
+    }
     if (tid < 4) cnt[tid] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,12 +60,23 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
         const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
         const double dzi = 1.0 / g.dx[2];
         double Tw[3] = {0, 0, 0}, qtw[3] = {0, 0, 0}, qvw[3] = {0, 0, 0}, thw[3] = {0, 0, 0}, bprev = 0.0;
+        // the next level's s and qt are loaded at every level (the level
+        // clamped to the chunk), so the wait for them counts only these loads
+        double s_nx = s[(long long)ch.kw * g.plane + col];
+        double q_nx = qtp[(long long)ch.kw * g.plane + col];
         for (int k = ch.kw; k < ch.ke; ++k) {
+            const int l = k - ch.kw;
             const bool own = k >= ch.k0 && k < ch.k1;
             const bool own_m = k - 1 >= ch.k0 && k - 1 < ch.k1;
             const long long o = (long long)k * g.plane + col;
-            const double p0 = a.p0[k];
-            double qt = qtp[o];
+            const double p0 = p0l[l];
+            const double sk = s_nx;
+            double qt = q_nx;
+            {
+                const long long on = (long long)min(k + 1, ch.ke - 1) * g.plane + col;
+                s_nx = s[on];
+                q_nx = qtp[on];
+            }
             if (a.clip) {
                 const double c = fmax(qt, 1e-11);
                 if (!(c == qt)) {
@@ -59,7 +85,7 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
                 }
             }
             double T, qv, ql, qi;
-            const int it = thermo::eos_sa(lt, p0, s[o], qt, &T, &qv, &ql, &qi, a.max_iter);
+            const int it = thermo::eos_sa(lt, p0, sk, qt, &T, &qv, &ql, &qi, a.max_iter);
             if (it > 0 && own) {
                 ctr[0] += 1;
                 ctr[1] += it;
@@ -67,8 +93,8 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
             }
             const double al = thermo::alpha_c(p0, T, qt, qv);
             double b = 0.0;
-            if (k < g.nzg - 1) b = thermo::buoyancy_c(a.alpha0_half[k], al);
-            const double ex = exner[k];
+            if (k < g.nzg - 1) b = thermo::buoyancy_c(a0l[l], al);
+            const double ex = exner[l];
             double thr = 0.0;
             if (k >= 1 && k < g.nzg - 1) thr = T * (1.0 - qt + PC_EPS_VI * qv) / ex;
             if (own) {
@@ -89,7 +115,7 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
             }
             // w tendency at k-1 for k-1 in [1, nzg-3): wt + interp_2(b[k-1], b[k])
             if (own_m && k >= 2 && k - 1 < g.nzg - 3) {
-                const double w0 = a.wt_over ? 0.0 : wt[o - g.plane];
+                const double w0 = WTO ? 0.0 : wt[o - g.plane];
                 wt[o - g.plane] = w0 + 0.5 * (bprev + b);
             }
             bprev = b;
@@ -99,7 +125,7 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
             thw[0] = thw[1]; thw[1] = thw[2]; thw[2] = thr;
             const int m = k - 1;
             if (bvfo && own_m && m >= 2 && m < g.nzg - 2)
-                bvfo[o - g.plane] = thermo::bvf_sa(lt, a.p0[m], Tw, qtw, qvw, thw, dzi);
+                bvfo[o - g.plane] = thermo::bvf_sa(lt, p0l[l - 1], Tw, qtw, qvw, thw, dzi);
         }
     }
     flush_counters(cnt, ctr, a.counters, tid);
@@ -171,8 +197,13 @@ extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, 
     const int nch = (g.nzg + a.kchunk - 1) / a.kchunk;
     dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, nch);
     hipStream_t st = (hipStream_t)stream;
-    if (precision == PYCLES_F64)
-        hipLaunchKernelGGL(thermo_sa_f64_kernel, grd, blk, 0, st, g, a);
+    if (precision == PYCLES_F64) {
+        const size_t smem = (size_t)(a.ntab + 3 * (a.kchunk + 2)) * sizeof(double);
+        if (a.wt_over)
+            hipLaunchKernelGGL(thermo_sa_f64_kernel<true>, grd, blk, smem, st, g, a);
+        else
+            hipLaunchKernelGGL(thermo_sa_f64_kernel<false>, grd, blk, smem, st, g, a);
+    }
     else if (precision == PYCLES_F32)
         return thermo_sa_f32_launch(g, a, grd, blk, st);
     else {
