@@ -12,9 +12,10 @@
 // evaluated by whole wavefronts (one row of 64 columns at one level) when any
 // of their cells is saturated; every lane stays in the loop for that reason.
 // Dynamic LDS: the pv* table in FP64, the chunk's p0 and base profiles (FP64),
-// the table in FP32, then the chunk's exner and alpha0 values.  Every per-level
-// value comes from LDS: a vector load of one would make each level wait for
-// all the loads and stores before it (the memory counter completes in order).
+// the table in FP32, then the chunk's 1/exner, alpha0, 1/alpha0 and 1/p0
+// (divisions by them are multiplications).  Every per-level value comes from
+// LDS: a vector load of one would make each level wait for all the loads and
+// stores before it (the memory counter completes in order).
 // WTO: the w tendency is overwritten, not added to (no read of it).
 struct NoSmem {};
 struct MlpSmem {
@@ -34,8 +35,10 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
     double *sbl = p0l + nl;
     double *qbl = sbl + nl;
     float *tabf = (float *)(qbl + nl);
-    float *exner = tabf + a.ntab;
-    float *a0l = exner + nl;
+    float *iexl = tabf + a.ntab;
+    float *a0l = iexl + nl;
+    float *ia0l = a0l + nl;
+    float *ip0l = ia0l + nl;
     __shared__ unsigned long long cnt[4];
     __shared__ typename std::conditional<MLP, MlpSmem, NoSmem>::type msm_store;
     MlpSmem *msm = reinterpret_cast<MlpSmem *>(&msm_store);
@@ -48,11 +51,13 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
     }
     for (int k = ch.kw + tid; k < ch.ke; k += nthr) {
         const int l = k - ch.kw;
-        exner[l] = (float)thermo::exner_c(a.p0[k]);
+        iexl[l] = (float)(1.0 / thermo::exner_c(a.p0[k]));
         p0l[l] = a.p0[k];
         sbl[l] = a.sbase ? a.sbase[k] : 0.0;
         qbl[l] = a.qbase ? a.qbase[k] : 0.0;
         a0l[l] = (float)a.alpha0_half[k];
+        ia0l[l] = (float)(1.0 / a.alpha0_half[k]);
+        ip0l[l] = (float)(1.0 / a.p0[k]);
     }
     if (tid < 4) cnt[tid] = 0;
     if (MLP) mlp::stage(a.mlp, &msm->w, tid, nthr);
@@ -124,15 +129,12 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
             if (T_guess <= 0.0f && a.max_iter > 0 && it >= a.max_iter) ctr[2] += 1;
             ctr[3] += fb;
         }
-        const float al = (float)PC_RD * T / p0f * (1.0f - qt + (float)PC_EPS_VI * qv);
-        const float ex = exner[l];
+        const float al = (float)PC_RD * T * ip0l[l] * (1.0f - qt + (float)PC_EPS_VI * qv);
+        const float iex = iexl[l];
         float b = 0.0f;
-        if (k < g.nzg - 1) {
-            const float a0 = a0l[l];
-            b = (float)PC_G * (al - a0) / a0;
-        }
+        if (k < g.nzg - 1) b = (float)PC_G * (al - a0l[l]) * ia0l[l];
         float thr = 0.0f;
-        if (k >= 1 && k < g.nzg - 1) thr = T * (1.0f - qt + (float)PC_EPS_VI * qv) / ex;
+        if (k >= 1 && k < g.nzg - 1) thr = T * (1.0f - qt + (float)PC_EPS_VI * qv) * iex;
         Tw[0] = Tw[1]; Tw[1] = Tw[2]; Tw[2] = T;
         qtw[0] = qtw[1]; qtw[1] = qtw[2]; qtw[2] = qt;
         qvw[0] = qvw[1]; qvw[1] = qvw[2]; qvw[2] = qv;
@@ -145,10 +147,11 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
             if (bo && k < g.nzg - 1) bo[o] = b;
             if (thro && k >= 1 && k < g.nzg - 1) thro[o] = thr;
             if (thlo) {
-                if (ql == 0.0f) thlo[o] = T / ex;
+                if (ql == 0.0f) thlo[o] = T * iex;
                 else {
                     const float L = thermo32::latent_heat(T);
-                    thlo[o] = T / ex * expf(-L * (ql / (1.0f - qt)) / (T * (float)PC_CPD));
+                    thlo[o] = T * iex *
+                              thermo32::fexp(-L * thermo32::fdiv(thermo32::fdiv(ql, 1.0f - qt), T * (float)PC_CPD));
                 }
             }
         }
@@ -165,18 +168,22 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
                 const float Lv = thermo32::latent_heat(Tk);
                 const float pvs = thermo32::lookup(ltf, Tk);
                 const float p0m = (float)p0l[l - 1];
-                const float rsl = (float)PC_EPS_V * pvs / (p0m - pvs);
+                using thermo32::fdiv;
+                using thermo32::frcp;
+                const float rsl = fdiv((float)PC_EPS_V * pvs, p0m - pvs);
                 const float cpm = (1.0f - qtw[1]) * (float)PC_CPD + qtw[1] * (float)PC_CPV;
-                const float gamma_w = (float)(PC_G / PC_CPD) * (1.0f / (1.0f - qtw[1])) *
-                                      (1.0f + Lv * rsl / ((float)PC_RD * Tk)) /
-                                      (cpm / (float)PC_CPD + Lv * Lv * ((float)PC_EPS_V + rsl) * rsl /
-                                                                 ((float)(PC_CPD * PC_RD) * Tk * Tk));
+                const float iqd = frcp(1.0f - qtw[1]);
+                const float iT = frcp(Tk);
+                const float lr = 1.0f + Lv * rsl * iT * (float)(1.0 / PC_RD);
+                const float gamma_w =
+                    fdiv((float)(PC_G / PC_CPD) * iqd * lr,
+                         cpm * (float)(1.0 / PC_CPD) +
+                             Lv * Lv * ((float)PC_EPS_V + rsl) * rsl * iT * iT * (float)(1.0 / (PC_CPD * PC_RD)));
                 const float dTdz = 0.5f * (Tw[2] - Tw[0]) * dzi;
                 const float dqtdz = 0.5f * (qtw[2] - qtw[0]) * dzi;
-                bv = (float)PC_G / Tk * (dTdz + gamma_w) * (1.0f + Lv * rsl / ((float)PC_RD * Tk)) -
-                     dqtdz / (1.0f - qtw[1]);
+                bv = (float)PC_G * iT * (dTdz + gamma_w) * lr - dqtdz * iqd;
             } else {
-                bv = (float)PC_G / thw[1] * 0.5f * (thw[2] - thw[0]) * dzi;
+                bv = (float)PC_G * thermo32::frcp(thw[1]) * 0.5f * (thw[2] - thw[0]) * dzi;
             }
             bvfo[o - g.plane] = bv;
         }
@@ -184,11 +191,11 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
     flush_counters(cnt, ctr, a.counters, tid);
 }
 
-// Dynamic LDS of the FP32 kernel: table (FP64 + FP32), three FP64 and two
+// Dynamic LDS of the FP32 kernel: table (FP64 + FP32), three FP64 and four
 // FP32 values per level of the chunk.
 static size_t f32_smem_bytes(const SAArgs &a) {
     return (size_t)a.ntab * (sizeof(double) + sizeof(float)) +
-           (size_t)(a.kchunk + 2) * (3 * sizeof(double) + 2 * sizeof(float));
+           (size_t)(a.kchunk + 2) * (3 * sizeof(double) + 4 * sizeof(float));
 }
 
 template <bool MLP>
