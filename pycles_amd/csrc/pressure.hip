@@ -315,7 +315,14 @@ template <typename TS>
 __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs, TS *cs,
                                                      const double *kx2, const double *ky2, const double *rho_h,
                                                      const double *dd, const double *a, const double *c, int nxh) {
-    constexpr int PF = 8;  // levels loaded ahead of the recurrence
+    // Levels loaded ahead of the recurrence.  One thread per column leaves
+    // about two waves per SIMD, so the loads in flight come from this depth.
+    // Every level of the main blocks issues the same loads and stores (the
+    // prefetched level clamped to the column; the tail of nz % PF levels runs
+    // after them): a load under a branch would make the wait for it count
+    // only from the branch's shorter path, i.e. wait for this level's load.
+    // FP64 scratch takes twice the registers: 8 levels measure faster there.
+    constexpr int PF = sizeof(TS) == 4 ? 16 : 8;
     const long long ncol = (long long)g.ny * nxh;
     const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
@@ -326,68 +333,73 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs
         for (int k = 0; k < nz; ++k) spec[(long long)k * ncol + col] = C2<TS>{TS(0), TS(0)};
         return;
     }
+    C2<TS> *xs2 = reinterpret_cast<C2<TS> *>(xs);
     const double kk = kx2[kx] + ky2[ky];
-    // forward sweep; the right-hand sides of the next PF levels are in flight
+    double sc = 0.0, xr = 0.0, xi = 0.0;
+    auto forward = [&](int k, C2<TS> z) {
+        const long long o = (long long)k * ncol + col;
+        const double b = rho_h[k] * kk - dd[k];
+        if (k == 0) {
+            sc = c[0] / b;
+            xr = (double)z.x / b;
+            xi = (double)z.y / b;
+        } else {
+            const double m = 1.0 / (b - a[k] * sc);
+            sc = c[k] * m;
+            xr = ((double)z.x - a[k] * xr) * m;
+            xi = ((double)z.y - a[k] * xi) * m;
+        }
+        cs[o] = (TS)sc;
+        xs2[o] = C2<TS>{(TS)xr, (TS)xi};
+    };
     C2<TS> zq[PF];
 #pragma unroll
-    for (int u = 0; u < PF; ++u) zq[u] = u < nz ? spec[(long long)u * ncol + col] : C2<TS>{TS(0), TS(0)};
-    double sc = 0.0, xr = 0.0, xi = 0.0;
-    for (int k0 = 0; k0 < nz; k0 += PF) {
+    for (int u = 0; u < PF; ++u) zq[u] = spec[(long long)min(u, nz - 1) * ncol + col];
+    int k0 = 0;
+    for (; k0 + PF <= nz; k0 += PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int k = k0 + u;
-            if (k < nz) {
-                const C2<TS> z = zq[u];
-                if (k + PF < nz) zq[u] = spec[(long long)(k + PF) * ncol + col];
-                const long long o = (long long)k * ncol + col;
-                const double b = rho_h[k] * kk - dd[k];
-                if (k == 0) {
-                    sc = c[0] / b;
-                    xr = (double)z.x / b;
-                    xi = (double)z.y / b;
-                } else {
-                    const double m = 1.0 / (b - a[k] * sc);
-                    sc = c[k] * m;
-                    xr = ((double)z.x - a[k] * xr) * m;
-                    xi = ((double)z.y - a[k] * xi) * m;
-                }
-                cs[o] = (TS)sc;
-                xs[2 * o] = (TS)xr;
-                xs[2 * o + 1] = (TS)xi;
-            }
+            const C2<TS> z = zq[u];
+            zq[u] = spec[(long long)min(k + PF, nz - 1) * ncol + col];
+            forward(k, z);
         }
     }
-    // back substitution; the scratch of the next PF levels down is in flight
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+        if (k0 + u < nz) forward(k0 + u, zq[u]);
+    // back substitution, from level nz - 2 down; the scratch of the next PF
+    // levels down is in flight
     spec[(long long)(nz - 1) * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
-    double sq[PF], rq[PF], iq[PF];
+    auto backward = [&](int k, TS s_, C2<TS> r_) {
+        xr = (double)r_.x - (double)s_ * xr;
+        xi = (double)r_.y - (double)s_ * xi;
+        spec[(long long)k * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
+    };
+    TS sq[PF];
+    C2<TS> rq[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-        const int k = nz - 2 - u;
-        if (k >= 0) {
-            const long long o = (long long)k * ncol + col;
-            sq[u] = (double)cs[o];
-            rq[u] = (double)xs[2 * o];
-            iq[u] = (double)xs[2 * o + 1];
-        }
+        const long long o = (long long)max(nz - 2 - u, 0) * ncol + col;
+        sq[u] = cs[o];
+        rq[u] = xs2[o];
     }
-    for (int k0 = nz - 2; k0 >= 0; k0 -= PF) {
+    int k1 = nz - 2;
+    for (; k1 - PF + 1 >= 0; k1 -= PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int k = k0 - u;
-            if (k >= 0) {
-                const double s_ = sq[u], r_ = rq[u], i_ = iq[u];
-                if (k - PF >= 0) {
-                    const long long on = (long long)(k - PF) * ncol + col;
-                    sq[u] = (double)cs[on];
-                    rq[u] = (double)xs[2 * on];
-                    iq[u] = (double)xs[2 * on + 1];
-                }
-                xr = r_ - s_ * xr;
-                xi = i_ - s_ * xi;
-                spec[(long long)k * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
-            }
+            const int k = k1 - u;
+            const TS s_ = sq[u];
+            const C2<TS> r_ = rq[u];
+            const long long on = (long long)max(k - PF, 0) * ncol + col;
+            sq[u] = cs[on];
+            rq[u] = xs2[on];
+            backward(k, s_, r_);
         }
     }
+#pragma unroll
+    for (int u = 0; u < PF; ++u)
+        if (k1 - u >= 0) backward(k1 - u, sq[u], rq[u]);
 }
 
 // Inverse x transform of 2S rows in pairs: Z[m] = X_a[m] + i X_b[m] with the
