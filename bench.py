@@ -58,7 +58,7 @@ KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, 
 # packed FP32, so half of it is their ceiling).
 KERNEL_FLOPS = {"adv_x": 755, "adv_y": 755, "adv_z": 755, "adv_z_rk3": 775}
 VALU_PEAK_TFLOPS = 157.3
-KERNEL_NAMES = {"thermo": "thermo_sa_f32_kernel<true>", "adv_x": "adv::adv_x<PolicyF32,5,5,2,true>",
+KERNEL_NAMES = {"thermo": "thermo_sa_f32_kernel<true,true>", "adv_x": "adv::adv_x<PolicyF32,5,5,2,true>",
                 "adv_y": "adv::adv_y<PolicyF32,5,5,2,true>", "adv_z": "adv::adv_z<PolicyF32,5,5,2,true>",
                 "rk3": "rk3_kernel<float,...>", "adv_z_rk3": "adv::adv_z<PolicyF32,5,5,2,true> (RK3 fused)"}
 
