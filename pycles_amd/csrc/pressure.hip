@@ -34,8 +34,9 @@ struct pycles_pressure_plan {
     Geo g;
     int precision;
     int nxh;
+    int nxp;         // row stride of the spectrum: nxh rounded up to 16 (128-byte rows in FP32)
     int gx;          // kx columns per y-FFT block
-    void *spec;      // [nz][ny][nxh] complex, field precision
+    void *spec;      // [nz][ny][nxp] complex, field precision (columns nxh..nxp-1 unused)
     void *xs;        // Thomas forward-sweep values, complex, storage type
     void *cs;        // Thomas forward-sweep coefficients, storage type
     void *twx;       // exp(-2 pi i m / nx), m < nx/2, interleaved (field precision)
@@ -182,7 +183,7 @@ template <typename TS>
 __global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const TS *v, const TS *w,
                                                        const double *alpha0, const double *alpha0_half,
                                                        const double *mean, TS *div_out, C2<TS> *spec,
-                                                       const C2<TS> *tw, int logx, int nxh, int logS) {
+                                                       const C2<TS> *tw, int logx, int nxh, int nxp, int logS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int S = 1 << logS;
     C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
@@ -256,8 +257,8 @@ __global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const
         const C2<TS> zm = buf[((g.nx - m) & (g.nx - 1)) * S + pb];
         const C2<TS> xa{TS(0.5) * (z.x + zm.x), TS(0.5) * (z.y - zm.y)};
         const C2<TS> xb{TS(0.5) * (z.y + zm.y), TS(-0.5) * (z.x - zm.x)};
-        spec[((long long)kk * g.ny + j0 + 2 * pb) * nxh + m] = xa;
-        spec[((long long)kk * g.ny + j0 + 2 * pb + 1) * nxh + m] = xb;
+        spec[((long long)kk * g.ny + j0 + 2 * pb) * nxp + m] = xa;
+        spec[((long long)kk * g.ny + j0 + 2 * pb + 1) * nxp + m] = xb;
     }
 }
 
@@ -265,13 +266,13 @@ __global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const
 // (inverse: times 1/ny).
 template <typename TS>
 __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C2<TS> *tw, int logy, int nxh,
-                                                   int gx, int loggx, int inverse) {
+                                                   int nxp, int gx, int loggx, int inverse) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
     const int kx0 = blockIdx.x * gx;
     const int kk = blockIdx.y;
     const int ncol = min(gx, nxh - kx0);
-    C2<TS> *lev = spec + (long long)kk * g.ny * nxh;
+    C2<TS> *lev = spec + (long long)kk * g.ny * nxp;
     // staged four elements per thread at a time, loads before LDS writes
     const int B = blockDim.x, nq = g.ny * gx;
     for (int q0 = threadIdx.x; q0 < nq; q0 += 4 * B) {
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
         for (int u = 0; u < 4; ++u) {
             const int q = q0 + u * B;
             const int y = q >> loggx, b = q & (gx - 1);
-            z[u] = (q < nq && b < ncol) ? lev[(long long)y * nxh + kx0 + b] : C2<TS>{TS(0), TS(0)};
+            z[u] = (q < nq && b < ncol) ? lev[(long long)y * nxp + kx0 + b] : C2<TS>{TS(0), TS(0)};
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
                 z.x *= sc;
                 z.y *= sc;
             }
-            lev[(long long)y * nxh + kx0 + bb] = z;
+            lev[(long long)y * nxp + kx0 + bb] = z;
         }
     }
 }
@@ -314,7 +315,8 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
 template <typename TS>
 __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs, TS *cs,
                                                      const double *kx2, const double *ky2, const double *rho_h,
-                                                     const double *dd, const double *a, const double *c, int nxh) {
+                                                     const double *dd, const double *a, const double *c, int nxh,
+                                                     int nxp) {
     // Levels loaded ahead of the recurrence.  One thread per column leaves
     // about two waves per SIMD, so the loads in flight come from this depth.
     // Every level of the main blocks issues the same loads and stores (the
@@ -323,11 +325,13 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs
     // only from the branch's shorter path, i.e. wait for this level's load.
     // FP64 scratch takes twice the registers: 8 levels measure faster there.
     constexpr int PF = sizeof(TS) == 4 ? 16 : 8;
-    const long long ncol = (long long)g.ny * nxh;
+    // one thread per column of the padded spectrum; the padding columns idle
+    const long long ncol = (long long)g.ny * nxp;
     const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
-    const int kx = (int)(col % nxh);
-    const int ky = (int)(col / nxh);
+    const int kx = (int)(col % nxp);
+    const int ky = (int)(col / nxp);
+    if (kx >= nxh) return;
     const int nz = g.nz;
     if (kx == 0 && ky == 0) {
         for (int k = 0; k < nz; ++k) spec[(long long)k * ncol + col] = C2<TS>{TS(0), TS(0)};
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs
 // together with the mirrored z ghost level (sym, as communicate_variable does).
 template <typename TS>
 __global__ __launch_bounds__(512) void ifftx_kernel(Geo g, const C2<TS> *spec, TS *p, const C2<TS> *tw, int logx,
-                                                    int nxh, int logS) {
+                                                    int nxh, int nxp, int logS) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int S = 1 << logS;
     C2<TS> *buf = reinterpret_cast<C2<TS> *>(smem_raw);
@@ -426,10 +430,10 @@ __global__ __launch_bounds__(512) void ifftx_kernel(Geo g, const C2<TS> *spec, T
             if (q < nq) {
                 const int m = q & (g.nx - 1);
                 const int pb = q >> logx;
-                const C2<TS> *ra = spec + ((long long)kk * g.ny + j0 + 2 * pb) * nxh;
+                const C2<TS> *ra = spec + ((long long)kk * g.ny + j0 + 2 * pb) * nxp;
                 const int mm = m < nxh ? m : g.nx - m;
                 xa[u] = ra[mm];
-                xb[u] = ra[nxh + mm];
+                xb[u] = ra[nxp + mm];
             }
         }
 #pragma unroll
@@ -605,6 +609,7 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     P->g = g;
     P->precision = precision;
     P->nxh = g.nx / 2 + 1;
+    P->nxp = (P->nxh + 15) & ~15;
     P->logx = logx;
     P->logy = logy;
     const size_t cs = precision == PYCLES_F64 ? 16 : 8;
@@ -615,7 +620,7 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
     while (gx > 1 && ((size_t)gx * g.ny / 8 > 1024 || (size_t)gx * g.ny * cs > 64 * 1024)) gx >>= 1;
     P->gx = gx;
     P->loggx = ilog2(gx);
-    const long long nspec = (long long)g.nz * g.ny * P->nxh;
+    const long long nspec = (long long)g.nz * g.ny * P->nxp;
     const int gw = g.gw, nz = g.nz;
     // host tables (same expressions as the reference initializers)
     std::vector<double> kx2(P->nxh), ky2(g.ny), rh(nz), dd(nz), a(nz), c(nz);
@@ -722,24 +727,24 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
     const int bx = S * g.nx / 8;
     hipLaunchKernelGGL(div_fftx_kernel<TS>, dim3(g.ny / (2 * S), g.nz), dim3(bx), rowsmem, st, g, (const TS *)u,
                        (const TS *)v, (const TS *)w, ref->alpha0, ref->alpha0_half, P->mean, (TS *)div,
-                       (C2<TS> *)P->spec, (const C2<TS> *)P->twx, P->logx, P->nxh, logS);
+                       (C2<TS> *)P->spec, (const C2<TS> *)P->twx, P->logx, P->nxh, P->nxp, logS);
     PYCLES_CHECK_LAUNCH();
     const size_t ysmem = sizeof(C2<TS>) * (size_t)g.ny * P->gx;
     const dim3 ygrid((P->nxh + P->gx - 1) / P->gx, g.nz);
     const int by = P->gx * g.ny / 8;
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
-                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 0);
+                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->nxp, P->gx, P->loggx, 0);
     PYCLES_CHECK_LAUNCH();
-    const long long ncol = (long long)g.ny * P->nxh;
+    const long long ncol = (long long)g.ny * P->nxp;
     hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
                        (C2<TS> *)P->spec, (TS *)P->xs, (TS *)P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c,
-                       P->nxh);
+                       P->nxh, P->nxp);
     PYCLES_CHECK_LAUNCH();
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
-                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->gx, P->loggx, 1);
+                       (const C2<TS> *)P->twy, P->logy, P->nxh, P->nxp, P->gx, P->loggx, 1);
     PYCLES_CHECK_LAUNCH();
     hipLaunchKernelGGL(ifftx_kernel<TS>, dim3(g.ny / (2 * S), g.nz), dim3(bx), rowsmem, st, g,
-                       (const C2<TS> *)P->spec, (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, logS);
+                       (const C2<TS> *)P->spec, (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, P->nxp, logS);
     PYCLES_CHECK_LAUNCH();
     if (cfl_out) hipLaunchKernelGGL(cfl_zero_kernel, dim3(1), dim3(1), 0, st, P->cfl);
     // z chunks (at least 2 gw levels each) so the grid holds ~16k waves
