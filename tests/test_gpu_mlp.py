@@ -182,6 +182,10 @@ def test_thermo_with_guess_matches_fp64_and_saves_iterations(case, trained, newt
     print(json.dumps({"newton": newton, "saturated": sat, "secant_iters_per_cell": secant_per_cell,
                       "newton_iters_per_cell": newton_per_cell, "fallback_fraction": fallback,
                       "plain_fallbacks": int(cp[3])}))
+    # the FP32 secant's results pass the FP64 residual check: a check that
+    # rejected good results would send every cell to the FP64 secant, with
+    # the same values and only the time lost
+    assert cp[3] <= 0.01 * sat
     # a cell whose guess fails the check also runs the FP32 secant
     assert newton <= newton_per_cell <= newton + fallback * (secant_per_cell + 5)
     assert secant_per_cell > newton_per_cell

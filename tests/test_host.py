@@ -116,3 +116,31 @@ def test_nan_cfl_stops_the_run():
     ts.cfl_dev = torch.tensor([0.35], dtype=torch.float64)
     ts.compute_cfl_max(None, None)
     assert ts.cfl_time_step() == 2.0
+
+
+def test_residual_check_logarithm_method():
+    """The FP32 thermodynamics checks its saturated results in FP64 with a
+    logarithm of its own (thermo_f32.h, log64): x = m 2^e, m in [sqrt(1/2),
+    sqrt(2)), ln x = e ln 2 + 2 atanh(t), t = (m - 1)/(m + 1), the series to
+    t^17.  Over the arguments the check meets (T/T~, pd/p~, pv/p~: 6e-6 to 20)
+    the truncated series must stay within 1e-14 of the library logarithm,
+    far inside the check's 3e-3 J/kg/K on terms of O(100) J/kg/K."""
+    import math
+    xs = np.exp(np.linspace(np.log(6e-6), np.log(20.0), 20001))
+    worst = 0.0
+    for x in xs:
+        m, e = math.frexp(float(x))
+        if m < 0.70710678118654752:
+            m *= 2.0
+            e -= 1
+        t = (m - 1.0) / (m + 1.0)
+        t2 = t * t
+        p = 1.0 / 17
+        for d in (15, 13, 11, 9, 7, 5, 3, 1):
+            p = p * t2 + 1.0 / d
+        worst = max(worst, abs(e * 0.69314718055994531 + 2.0 * t * p - math.log(x)))
+    assert worst < 1e-14
+    # the truncation is what bounds it: one term fewer is ~40x worse
+    m = math.sqrt(2.0) - 1e-12
+    t = (m - 1.0) / (m + 1.0)
+    assert t ** 19 / 19 * 2 < 1e-15 < t ** 17 / 17 * 2
