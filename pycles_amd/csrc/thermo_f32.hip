@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    unsigned long long ctr[4] = {0, 0, 0, 0};
+    unsigned ctr[4] = {0, 0, 0, 0};  // one thread's counts fit in 32 bits
     const bool active = i < g.nx && j < g.ny;
     const long long col = active ? (long long)j * g.nx + i : 0;
     float *s = (float *)a.s;
@@ -188,7 +188,8 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
             bvfo[o - g.plane] = bv;
         }
     }
-    flush_counters(cnt, ctr, a.counters, tid);
+    const unsigned long long ctr64[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    flush_counters(cnt, ctr64, a.counters, tid);
 }
 
 // Dynamic LDS of the FP32 kernel: table (FP64 + FP32), three FP64 and four
