@@ -22,10 +22,10 @@ struct MlpSmem {
     mlp::Weights w;
 };
 
-// Waves per SIMD: the plain kernel is bound by memory and dependency
-// latency; at 5 waves it keeps its registers (at 6 its spill reloads, which
-// wait for every outstanding load and store, cost more than the extra wave
-// gains); the network's registers keep the learned kernel at 4.
+// Waves per SIMD: at 5 the plain kernel keeps its registers (at 6 its spill
+// reloads, which wait for every outstanding load and store, cost more than
+// the extra wave gains); the network's registers keep the learned kernel at 4
+// (at 5 it spills and runs slower).
 template <bool MLP, bool WTO>
 __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, SAArgs a) {
     extern __shared__ double dyn[];
