@@ -176,10 +176,12 @@ def test_rk3_stage_and_bcs_bitwise(gpu, rk):
     assert float(sim.PV.tendencies.abs().max()) == 0.0
 
 
-@pytest.mark.parametrize("nx,ny", [(32, 16), (8, 64), (16, 128), (64, 8), (256, 32)])
-def test_pressure_parity(gpu, nx, ny):
-    # nx, ny cover first Stockham passes of radix 2, 4 and 8 in both directions
-    nl, Gr, RS, Th, fields = host_case(nx, ny, 24, seed=9)
+@pytest.mark.parametrize("nx,ny,nz", [(32, 16, 24), (8, 64, 24), (16, 128, 24), (64, 8, 24), (256, 32, 24),
+                                      (32, 16, 13), (16, 8, 5)])
+def test_pressure_parity(gpu, nx, ny, nz):
+    # nx, ny cover first Stockham passes of radix 2, 4 and 8 in both directions;
+    # nz = 13 and 5 leave tails after the Thomas sweeps' blocks of 8 levels
+    nl, Gr, RS, Th, fields = host_case(nx, ny, nz, seed=9)
     fields = _swirl(Gr, fields, amp=2.0)
     sim = _gpu_sim(nl, Gr, RS, Th, fields)
     sim.Pr.update(sim.Gr, sim.Ref, sim.DV, sim.PV, sim.Pa)
