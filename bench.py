@@ -47,9 +47,10 @@ METRIC = "Mcell-updates/s per RK3 step: FP32+learned vs plain FP64 at <=1e-5 rel
 #           copy (written in stage 0, read in stages 1, 2); five variables  20 x 4
 #           (only when the RK3 update is not fused into the z pass)
 #   adv_z_rk3  the z pass with the RK3 stage fused: reads the five fields and
-#           their tendencies, reads (stages 1, 2) or writes (stage 0) the stage
-#           copies, writes the five new values                   20 x 4
-KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, "adv_z_rk3": 80}
+#           their tendencies, writes the five new values (stage 0: 15 x 4);
+#           stages 1, 2 also read the stage copies (20 x 4); the launch time
+#           is the mean over the three stages, so are the bytes   (15 + 20 + 20) / 3 x 4
+KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, "adv_z_rk3": 220 / 3}
 # Algorithmic flops per interior cell of the advection passes: WENO5 scalar and
 # momentum advection is ~2265 flop per cell per stage over the three directions
 # (2 scalar and 9 momentum face fluxes, interpolation and divergence), split
@@ -187,7 +188,7 @@ def per_kernel(phases, steps):
         if k not in phases:
             continue
         ms = phases[k] / (3 * steps)
-        out[k] = {"ms_per_launch": round(ms, 4), "bytes_per_cell": nbytes}
+        out[k] = {"ms_per_launch": round(ms, 4), "bytes_per_cell": round(nbytes, 2)}
     return out
 
 
