@@ -45,3 +45,34 @@ def test_fp32_tracks_fp64(gpu, order):
         a = s32.PV.get_device_layout(n)
         b = s64.PV.get_device_layout(n)
         assert np.max(np.abs(a - b)) < 1e-3 * max(1.0, np.max(np.abs(b))), n
+
+
+def test_fp32_thermo_clip_and_saturation(gpu):
+    """One FP32 thermodynamics update on a state with a saturated share and a
+    level of negative qt (clip_qt, thermodynamics_sa.h:213-220): T against the
+    FP64 oracle within 1e-5, and the clipped level's stored qt at 1e-11 (its
+    FP32 deviation from the level profile resolves ~1e-9)."""
+    import json
+    from pycles_amd.Simulation3d import Simulation3d
+    from tests.helpers import oracle_sim
+    from tests.test_gpu_parity import _saturating
+    nl, Gr, RS, Th, fields = host_case(32, 24, 40, seed=4)
+    fields = _saturating(Gr, fields)
+    nl = json.loads(json.dumps(nl))
+    nl["mi355x"]["precision"] = "f32"
+    sim = Simulation3d().initialize(nl, Gr=Gr, Ref=RS, Th=Th, fields=fields)
+    sim.Th.max_iter = 0
+    sim.Th.update(sim.Gr, sim.Ref, sim.PV, sim.DV)
+    torch.cuda.synchronize()
+    osim = oracle_sim(Gr, RS, Th, fields, True)
+    osim.thermo_update()
+    gw = Gr.dims.gw
+    assert osim.sa_stats.saturated > 0.05 * osim.npg
+    T_o = Gr.to_device_layout(osim.dv["temperature"])[gw:-gw]
+    T_g = sim.DV.get_device_layout("temperature")[gw:-gw]
+    assert np.max(np.abs(T_g - T_o) / T_o) < 1e-5
+    qt = sim.PV.get_device_layout("qt")
+    assert np.all(np.abs(qt[gw + 2] - 1e-11) < 2e-9)
+    others = np.delete(np.arange(gw, qt.shape[0] - gw), 2)
+    qt_o = fields["qt"]
+    assert np.max(np.abs(qt[others] - qt_o[others]) / np.abs(qt_o[others])) < 1e-6
