@@ -97,10 +97,11 @@ def test_forward_and_packed_inference_match_oracle(trained):
                       "mfma_vs_oracle_max_K": float(np.abs(d).max())}))
     # The hidden activations are rounded to bf16 on both sides; the device's
     # fast tanh moves a few of them across a bf16 rounding boundary (one step
-    # is 2^-8 relative), which shifts T by up to a few 1e-2 K.  A wrong lane
-    # layout of the matrix-core operands would give errors of kelvins.
-    assert np.sqrt(np.mean(d ** 2)) < 5e-3
-    assert np.abs(d).max() < 0.1
+    # is 2^-8 relative), which shifts T by up to a few 1e-2 K (measured:
+    # 9e-4 K RMS, 0.03 K at most).  A wrong lane layout or K order of the
+    # matrix-core operands would give errors of kelvins.
+    assert np.sqrt(np.mean(d ** 2)) < 2.5e-3
+    assert np.abs(d).max() < 0.06
 
 
 def test_first_adam_step(case):
