@@ -59,3 +59,24 @@ def test_bf16_rounding():
             assert abs(q - v) <= abs(v) * two ** -8
             m = np.frexp(q)[0] * 2 ** 8           # at most 8 significant bits
             assert m == np.round(m)
+
+
+def test_bf16_inference_is_the_forward_pass_to_bf16_precision():
+    """infer_bf16 (what the packed matrix-core inference computes) differs from
+    the float64 forward pass only by the bf16 roundings of W1, TANH_C W2, W3
+    and the first hidden activations: each is 2^-9 relative at most, so the
+    output moves by a few 1e-3 of its scale and not more; and TANH_C is the
+    FP32 value of 2 log2(e), the factor of tanh(x) = 1 - 2/(2^(TANH_C x) + 1)."""
+    rng = np.random.default_rng(4)
+    p = _params(rng)
+    x = rng.normal(size=(4096, 3))
+    y64 = M.forward(p, x)[0]
+    ybf = M.infer_bf16(p, x)
+    scale = np.abs(y64).max()
+    err = np.abs(ybf - y64)
+    assert err.max() < 1e-2 * scale
+    assert np.sqrt(np.mean(err ** 2)) < 3e-3 * scale
+    assert err.max() > 0.0  # the roundings are there
+    assert abs(M.TANH_C - 2.0 / np.log(2.0)) < 1e-7 * M.TANH_C
+    u = np.linspace(-3.0, 3.0, 13)
+    assert np.allclose(1.0 - 2.0 / (np.exp2(M.TANH_C * u) + 1.0), np.tanh(u), rtol=0, atol=1e-6)
