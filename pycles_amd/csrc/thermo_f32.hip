@@ -17,6 +17,17 @@
 // LDS: a vector load of one would make each level wait for all the loads and
 // stores before it (the memory counter completes in order).
 // WTO: the w tendency is overwritten, not added to (no read of it).
+
+// A copy of a uniform pointer in an SGPR pair of its own, opaque to the
+// compiler (a register move).  The kernel arguments arrive as one 16-SGPR
+// load; the kernel needs more SGPRs than it has, and once that tuple is
+// spilled (to VGPR lanes) every use of one of its pointers restores all 16
+// registers, one v_readlane each, in every store block of the level loop.
+template <typename T> __device__ __forceinline__ T *sreg(T *p) {
+    T *q;
+    asm volatile("s_mov_b64 %0, %1" : "=s"(q) : "s"(p));
+    return q;
+}
 struct NoSmem {};
 struct MlpSmem {
     mlp::Weights w;
@@ -67,11 +78,12 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
     unsigned ctr[4] = {0, 0, 0, 0};  // one thread's counts fit in 32 bits
     const bool active = i < g.nx && j < g.ny;
     const long long col = active ? (long long)j * g.nx + i : 0;
-    float *s = (float *)a.s;
-    float *qtp = (float *)a.qt;
-    float *wt = (float *)a.wt;
-    float *To = (float *)a.T, *qvo = (float *)a.qv, *qlo = (float *)a.ql, *alo = (float *)a.alpha;
-    float *bo = (float *)a.buoy, *thro = (float *)a.thr, *bvfo = (float *)a.bvf, *thlo = (float *)a.thl;
+    float *s = sreg((float *)a.s);
+    float *qtp = sreg((float *)a.qt);
+    float *wt = sreg((float *)a.wt);
+    float *To = sreg((float *)a.T), *qvo = sreg((float *)a.qv), *qlo = sreg((float *)a.ql);
+    float *alo = sreg((float *)a.alpha), *bo = sreg((float *)a.buoy), *thro = sreg((float *)a.thr);
+    float *bvfo = sreg((float *)a.bvf), *thlo = sreg((float *)a.thl);
     const thermo32::TableF ltf{tabf, (float)a.x_min, (float)a.dx, (float)a.dxi, a.ntab};
     const thermo::Table lt{tab, a.x_min, a.dx, a.dxi, a.ntab};
     const float dzi = (float)(1.0 / g.dx[2]);
