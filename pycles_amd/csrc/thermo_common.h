@@ -32,6 +32,18 @@ __device__ __forceinline__ void flush_counters(unsigned long long *cnt, const un
     if (tid < 4 && cnt[tid]) atomicAdd(&out[tid], cnt[tid]);
 }
 
+// A copy of a uniform pointer in an SGPR pair of its own, opaque to the
+// compiler (a register move).  The kernel arguments arrive in wide scalar
+// loads; the thermodynamics kernels need more SGPRs than they have, and once
+// such a tuple is spilled (to VGPR lanes) every use of one of its pointers
+// restores the whole tuple, one v_readlane per register, in every store
+// block of the level loop.
+template <typename T> __device__ __forceinline__ T *sreg(T *p) {
+    T *q;
+    asm volatile("s_mov_b64 %0, %1" : "=s"(q) : "s"(p));
+    return q;
+}
+
 // z chunk of this block: owned [k0, k1), marched [kw, ke)
 struct Chunk {
     int k0, k1, kw, ke;

@@ -17,17 +17,7 @@
 // LDS: a vector load of one would make each level wait for all the loads and
 // stores before it (the memory counter completes in order).
 // WTO: the w tendency is overwritten, not added to (no read of it).
-
-// A copy of a uniform pointer in an SGPR pair of its own, opaque to the
-// compiler (a register move).  The kernel arguments arrive as one 16-SGPR
-// load; the kernel needs more SGPRs than it has, and once that tuple is
-// spilled (to VGPR lanes) every use of one of its pointers restores all 16
-// registers, one v_readlane each, in every store block of the level loop.
-template <typename T> __device__ __forceinline__ T *sreg(T *p) {
-    T *q;
-    asm volatile("s_mov_b64 %0, %1" : "=s"(q) : "s"(p));
-    return q;
-}
+// The field pointers are copied into SGPR pairs of their own (sreg).
 struct NoSmem {};
 struct MlpSmem {
     mlp::Weights w;
