@@ -184,7 +184,8 @@ int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const 
 /* ---- pressure projection ----------------------------------------------- */
 typedef struct pycles_pressure_plan pycles_pressure_plan;
 /* Set-up: modified wavenumbers, tridiagonal coefficients and the spectral
- * workspace (PressureFFTParallel.initialize, PressureFFTParallel.pyx:27-148). */
+ * workspace (PressureFFTParallel.initialize, PressureFFTParallel.pyx:27-148).
+ * nx and ny are powers of two in [8, 4096]; nz is in [2, 2048]. */
 int pycles_pressure_plan_create(const pycles_dims *dims, const double *rho0_host,
                                 const double *rho0_half_host, int precision,
                                 pycles_pressure_plan **plan);

@@ -305,6 +305,10 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
     }
 }
 
+// Largest nz of the pressure solve: the FP32 form's Thomas kernel holds four
+// FP64 level profiles in LDS (64 KB at this size).
+constexpr int PRESSURE_MAX_NZ = 2048;
+
 // One thread per (kx, ky) column: TDMA of SparseSolvers.pyx:24-43 on the real
 // and imaginary parts, with the diagonal of compute_diagonal
 // (PressureFFTParallel.pyx:129-148); the (0,0) mode is set to zero (:196-200).
@@ -404,6 +408,152 @@ __global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs
 #pragma unroll
     for (int u = 0; u < PF; ++u)
         if (k1 - u >= 0) backward(k1 - u, sq[u], rq[u]);
+}
+
+// 1/d in FP64 from the hardware reciprocal and two Newton steps (within an
+// ulp or two of the division; no special values occur: d is a pivot of a
+// diagonally dominant system).
+__device__ __forceinline__ double rcp64(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+// The FP32 form's Thomas solve (SparseSolvers.pyx:24-43, as thomas_kernel)
+// without per-level scratch.  The forward sweep streams the right-hand sides
+// and keeps only the FP64 carry (c', x'r, x'i) at the last level of every
+// S-level segment (3 doubles per column per segment, in the scratch array).
+// The back substitution walks the segments from the top: it recomputes a
+// segment's forward values from the carry below it and the right-hand sides,
+// which are still in place (the solution overwrites a segment only after its
+// forward values are recomputed), then substitutes down the segment.  The
+// next segment's right-hand sides and carry are loaded while the current one
+// is worked on.  24 B per column-level instead of 40.  Both sweeps evaluate
+// the same FP64 expressions, so the recomputed values are the forward
+// sweep's, bit for bit; the reciprocal is rcp64, not an IEEE division.
+template <int S>
+__global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *spec, double *ck, const double *kx2,
+                                                          const double *ky2, const double *rho_h, const double *dd,
+                                                          const double *a, const double *c, int nxh, int nxp) {
+    // The level profiles (rho_h, dd, a, c) in LDS, 4 nz doubles: read from
+    // global memory in the level loops they are vector loads (the stores
+    // keep the compiler from the scalar cache), and each wait for one would
+    // also wait for every older load and store, right-hand sides included.
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *lv = reinterpret_cast<double *>(smem_raw);
+    const int nz = g.nz;
+    for (int t = threadIdx.x; t < nz; t += blockDim.x) {
+        lv[t] = rho_h[t];
+        lv[nz + t] = dd[t];
+        lv[2 * nz + t] = a[t];
+        lv[3 * nz + t] = c[t];
+    }
+    __syncthreads();
+    const double *lrho = lv, *ldd = lv + nz, *la = lv + 2 * nz, *lc = lv + 3 * nz;
+    const long long ncol = (long long)g.ny * nxp;
+    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const int kx = (int)(col % nxp);
+    const int ky = (int)(col / nxp);
+    if (kx >= nxh) return;
+    if (kx == 0 && ky == 0) {
+        for (int k = 0; k < nz; ++k) spec[(long long)k * ncol + col] = C2<float>{0.0f, 0.0f};
+        return;
+    }
+    const double kk = kx2[kx] + ky2[ky];
+    auto at = [&](int k) { return spec + (long long)k * ncol + col; };
+    // one forward step: the carry of level k - 1 to level k
+    auto fstep = [&](int k, C2<float> z, double &sc, double &xr, double &xi) {
+        const double b = lrho[k] * kk - ldd[k];
+        if (k == 0) {
+            const double ib = rcp64(b);
+            sc = lc[0] * ib;
+            xr = (double)z.x * ib;
+            xi = (double)z.y * ib;
+        } else {
+            const double m = rcp64(b - la[k] * sc);
+            sc = lc[k] * m;
+            xr = ((double)z.x - la[k] * xr) * m;
+            xi = ((double)z.y - la[k] * xi) * m;
+        }
+    };
+    // forward sweep: right-hand sides S levels ahead, loaded at every level
+    // (clamped), a carry stored at the end of every whole segment
+    double sc = 0.0, xr = 0.0, xi = 0.0;
+    C2<float> zq[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) zq[u] = *at(min(u, nz - 1));
+    int k0 = 0;
+    for (; k0 + S <= nz; k0 += S) {
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int k = k0 + u;
+            const C2<float> z = zq[u];
+            zq[u] = *at(min(k + S, nz - 1));
+            fstep(k, z, sc, xr, xi);
+        }
+        const long long q = 3LL * (k0 / S) * ncol + col;
+        ck[q] = sc;
+        ck[q + ncol] = xr;
+        ck[q + 2 * ncol] = xi;
+    }
+#pragma unroll
+    for (int u = 0; u < S; ++u)
+        if (k0 + u < nz) fstep(k0 + u, zq[u], sc, xr, xi);
+    // the top level's solution is its forward value
+    *at(nz - 1) = C2<float>{(float)xr, (float)xi};
+    // back substitution, segment by segment from the top; (xr, xi) carries
+    // the solution of the level above, in FP64.  Segment sg needs the carry
+    // of segment sg - 1 (none for sg = 0, whose first step starts afresh);
+    // the loads of the next segment down use a clamped index, and with a
+    // single segment no carry is read.
+    const int nseg = (nz + S - 1) / S;
+    const int top = nz - 2 < 0 ? 0 : nz - 2;  // the highest level still holding its right-hand side
+    C2<float> zs[S];
+    double csc = 0.0, cxr = 0.0, cxi = 0.0;
+    auto load_seg = [&](int sg, C2<float> *z, double &lsc, double &lxr, double &lxi) {
+#pragma unroll
+        for (int u = 0; u < S; ++u) z[u] = *at(min(sg * S + u, top));
+        if (nseg > 1) {
+            const long long q = 3LL * max(sg - 1, 0) * ncol + col;
+            lsc = ck[q];
+            lxr = ck[q + ncol];
+            lxi = ck[q + 2 * ncol];
+        }
+    };
+    load_seg(nseg - 1, zs, csc, cxr, cxi);
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int lo = sg * S;
+        C2<float> zn[S];
+        double nsc = 0.0, nxr = 0.0, nxi = 0.0;
+        load_seg(max(sg - 1, 0), zn, nsc, nxr, nxi);
+        float sT[S], rT[S], iT[S];
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            if (lo + u < nz - 1) {
+                fstep(lo + u, zs[u], csc, cxr, cxi);
+                sT[u] = (float)csc;
+                rT[u] = (float)cxr;
+                iT[u] = (float)cxi;
+            }
+        }
+#pragma unroll
+        for (int u = S - 1; u >= 0; --u) {
+            const int k = lo + u;
+            if (k < nz - 1) {
+                xr = (double)rT[u] - (double)sT[u] * xr;
+                xi = (double)iT[u] - (double)sT[u] * xi;
+                *at(k) = C2<float>{(float)xr, (float)xi};
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < S; ++u) zs[u] = zn[u];
+        csc = nsc;
+        cxr = nxr;
+        cxi = nxi;
+    }
 }
 
 // Inverse x transform of 2S rows in pairs: Z[m] = X_a[m] + i X_b[m] with the
@@ -600,8 +750,8 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
         pycles_set_error("pressure: nx, ny must be powers of two in [8, 4096] (got %d, %d)", g.nx, g.ny);
         return PYCLES_ERR_UNSUPPORTED;
     }
-    if (g.nz < 2) {
-        pycles_set_error("pressure: nz must be at least 2");
+    if (g.nz < 2 || g.nz > PRESSURE_MAX_NZ) {
+        pycles_set_error("pressure: nz must be in [2, %d] (got %d)", PRESSURE_MAX_NZ, g.nz);
         return PYCLES_ERR_UNSUPPORTED;
     }
     pycles_pressure_plan *P = new pycles_pressure_plan();
@@ -736,9 +886,15 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->nxp, P->gx, P->loggx, 0);
     PYCLES_CHECK_LAUNCH();
     const long long ncol = (long long)g.ny * P->nxp;
-    hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
-                       (C2<TS> *)P->spec, (TS *)P->xs, (TS *)P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c,
-                       P->nxh, P->nxp);
+    if constexpr (sizeof(TS) == 4)
+        hipLaunchKernelGGL(thomas_ckpt_kernel<16>, dim3((unsigned)((ncol + 255) / 256)), dim3(256),
+                           4 * sizeof(double) * (size_t)g.nz, st, g,
+                           (C2<float> *)P->spec, (double *)P->xs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c,
+                           P->nxh, P->nxp);
+    else
+        hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
+                           (C2<TS> *)P->spec, (TS *)P->xs, (TS *)P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a,
+                           P->c, P->nxh, P->nxp);
     PYCLES_CHECK_LAUNCH();
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->nxp, P->gx, P->loggx, 1);

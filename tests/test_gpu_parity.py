@@ -206,7 +206,7 @@ def test_pressure_parity(gpu, nx, ny, nz):
     assert np.abs(div_g).max() <= max(10 * np.abs(div_o).max(), 1e-13 * scale)
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(32, 16, 24), (64, 32, 23), (16, 128, 9)])
+@pytest.mark.parametrize("nx,ny,nz", [(32, 16, 24), (64, 32, 23), (16, 128, 9), (16, 16, 16), (32, 16, 33)])
 def test_pressure_f32(gpu, nx, ny, nz):
     """FP32 projection (FP32 FFTs; tridiagonal solve with FP64 recurrence and
     FP32 scratch) against the FP64 oracle, and the divergence it leaves."""
