@@ -37,8 +37,7 @@ struct pycles_pressure_plan {
     int nxp;         // row stride of the spectrum: nxh rounded up to 16 (128-byte rows in FP32)
     int gx;          // kx columns per y-FFT block
     void *spec;      // [nz][ny][nxp] complex, field precision (columns nxh..nxp-1 unused)
-    void *xs;        // Thomas forward-sweep values, complex, storage type
-    void *cs;        // Thomas forward-sweep coefficients, storage type
+    double *ck;      // Thomas segment carries: [nz / S][3][ny][nxp] (S levels per segment)
     void *twx;       // exp(-2 pi i m / nx), m < nx/2, interleaved (field precision)
     void *twy;
     double *kx2, *ky2, *rho_h, *dd, *a, *c;
@@ -305,110 +304,13 @@ __global__ __launch_bounds__(1024) void ffty_kernel(Geo g, C2<TS> *spec, const C
     }
 }
 
-// Largest nz of the pressure solve: the FP32 form's Thomas kernel holds four
-// FP64 level profiles in LDS (64 KB at this size).
+// Largest nz of the pressure solve: the Thomas kernel holds four FP64 level
+// profiles in LDS (64 KB at this size).
 constexpr int PRESSURE_MAX_NZ = 2048;
 
-// One thread per (kx, ky) column: TDMA of SparseSolvers.pyx:24-43 on the real
-// and imaginary parts, with the diagonal of compute_diagonal
-// (PressureFFTParallel.pyx:129-148); the (0,0) mode is set to zero (:196-200).
-// The recurrence runs in FP64; its scratch (coefficients and forward-swept
-// values, written by the forward sweep and read back by the back
-// substitution) is stored in the storage type TS: FP64 for the FP64 form,
-// FP32 for the FP32 form, whose right-hand side and solution are FP32 already.
-template <typename TS>
-__global__ __launch_bounds__(256) void thomas_kernel(Geo g, C2<TS> *spec, TS *xs, TS *cs,
-                                                     const double *kx2, const double *ky2, const double *rho_h,
-                                                     const double *dd, const double *a, const double *c, int nxh,
-                                                     int nxp) {
-    // Levels loaded ahead of the recurrence.  One thread per column leaves
-    // about two waves per SIMD, so the loads in flight come from this depth.
-    // Every level of the main blocks issues the same loads and stores (the
-    // prefetched level clamped to the column; the tail of nz % PF levels runs
-    // after them): a load under a branch would make the wait for it count
-    // only from the branch's shorter path, i.e. wait for this level's load.
-    // FP64 scratch takes twice the registers: 8 levels measure faster there.
-    constexpr int PF = sizeof(TS) == 4 ? 16 : 8;
-    // one thread per column of the padded spectrum; the padding columns idle
-    const long long ncol = (long long)g.ny * nxp;
-    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncol) return;
-    const int kx = (int)(col % nxp);
-    const int ky = (int)(col / nxp);
-    if (kx >= nxh) return;
-    const int nz = g.nz;
-    if (kx == 0 && ky == 0) {
-        for (int k = 0; k < nz; ++k) spec[(long long)k * ncol + col] = C2<TS>{TS(0), TS(0)};
-        return;
-    }
-    C2<TS> *xs2 = reinterpret_cast<C2<TS> *>(xs);
-    const double kk = kx2[kx] + ky2[ky];
-    double sc = 0.0, xr = 0.0, xi = 0.0;
-    auto forward = [&](int k, C2<TS> z) {
-        const long long o = (long long)k * ncol + col;
-        const double b = rho_h[k] * kk - dd[k];
-        if (k == 0) {
-            sc = c[0] / b;
-            xr = (double)z.x / b;
-            xi = (double)z.y / b;
-        } else {
-            const double m = 1.0 / (b - a[k] * sc);
-            sc = c[k] * m;
-            xr = ((double)z.x - a[k] * xr) * m;
-            xi = ((double)z.y - a[k] * xi) * m;
-        }
-        cs[o] = (TS)sc;
-        xs2[o] = C2<TS>{(TS)xr, (TS)xi};
-    };
-    C2<TS> zq[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u) zq[u] = spec[(long long)min(u, nz - 1) * ncol + col];
-    int k0 = 0;
-    for (; k0 + PF <= nz; k0 += PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int k = k0 + u;
-            const C2<TS> z = zq[u];
-            zq[u] = spec[(long long)min(k + PF, nz - 1) * ncol + col];
-            forward(k, z);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < PF; ++u)
-        if (k0 + u < nz) forward(k0 + u, zq[u]);
-    // back substitution, from level nz - 2 down; the scratch of the next PF
-    // levels down is in flight
-    spec[(long long)(nz - 1) * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
-    auto backward = [&](int k, TS s_, C2<TS> r_) {
-        xr = (double)r_.x - (double)s_ * xr;
-        xi = (double)r_.y - (double)s_ * xi;
-        spec[(long long)k * ncol + col] = C2<TS>{(TS)xr, (TS)xi};
-    };
-    TS sq[PF];
-    C2<TS> rq[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-        const long long o = (long long)max(nz - 2 - u, 0) * ncol + col;
-        sq[u] = cs[o];
-        rq[u] = xs2[o];
-    }
-    int k1 = nz - 2;
-    for (; k1 - PF + 1 >= 0; k1 -= PF) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const int k = k1 - u;
-            const TS s_ = sq[u];
-            const C2<TS> r_ = rq[u];
-            const long long on = (long long)max(k - PF, 0) * ncol + col;
-            sq[u] = cs[on];
-            rq[u] = xs2[on];
-            backward(k, s_, r_);
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < PF; ++u)
-        if (k1 - u >= 0) backward(k1 - u, sq[u], rq[u]);
-}
+// Levels per segment of the checkpointed Thomas solve: the right-hand sides
+// of two segments are held in registers, twice the registers per level in FP64.
+template <typename TS> constexpr int thomas_seg() { return sizeof(TS) == 4 ? 16 : 8; }
 
 // 1/d in FP64 from the hardware reciprocal and two Newton steps (within an
 // ulp or two of the division; no special values occur: d is a pivot of a
@@ -421,22 +323,29 @@ __device__ __forceinline__ double rcp64(double d) {
     return __builtin_fma(r, e, r);
 }
 
-// The FP32 form's Thomas solve (SparseSolvers.pyx:24-43, as thomas_kernel)
-// without per-level scratch.  The forward sweep streams the right-hand sides
-// and keeps only the FP64 carry (c', x'r, x'i) at the last level of every
-// S-level segment (3 doubles per column per segment, in the scratch array).
-// The back substitution walks the segments from the top: it recomputes a
-// segment's forward values from the carry below it and the right-hand sides,
-// which are still in place (the solution overwrites a segment only after its
-// forward values are recomputed), then substitutes down the segment.  The
-// next segment's right-hand sides and carry are loaded while the current one
-// is worked on.  24 B per column-level instead of 40.  Both sweeps evaluate
-// the same FP64 expressions, so the recomputed values are the forward
-// sweep's, bit for bit; the reciprocal is rcp64, not an IEEE division.
-template <int S>
-__global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *spec, double *ck, const double *kx2,
-                                                          const double *ky2, const double *rho_h, const double *dd,
-                                                          const double *a, const double *c, int nxh, int nxp) {
+// One thread per (kx, ky) column: TDMA of SparseSolvers.pyx:24-43 on the real
+// and imaginary parts, with the diagonal of compute_diagonal
+// (PressureFFTParallel.pyx:129-148); the (0,0) mode is set to zero (:196-200).
+// The recurrence runs in FP64 whatever the storage type TS.  No per-level
+// scratch: the forward sweep streams the right-hand sides and keeps only the
+// FP64 carry (c', x'r, x'i) at the last level of every S-level segment (3
+// doubles per column per segment, in ck).  The back substitution walks the
+// segments from the top: it recomputes a segment's forward values from the
+// carry below it and the right-hand sides, which are still in place (the
+// solution overwrites a segment only after its forward values are
+// recomputed), then substitutes down the segment.  The next segment's
+// right-hand sides and carry are loaded while the current one is worked on.
+// Per column-level: the right-hand side read twice and the solution written
+// (24 B in FP32, 48 B in FP64), where a stored scratch of forward values
+// would add its write and read back (40 and 80 B).  Both sweeps evaluate the
+// same FP64 expressions, so the recomputed values are the forward sweep's,
+// bit for bit; the reciprocal is rcp64, not an IEEE division.  The forward
+// values enter the back substitution rounded to TS, as the reference's
+// in-place arrays would hold them in that precision.
+template <typename TS, int S>
+__global__ __launch_bounds__(256, 3) void thomas_kernel(Geo g, C2<TS> *spec, double *ck, const double *kx2,
+                                                        const double *ky2, const double *rho_h, const double *dd,
+                                                        const double *a, const double *c, int nxh, int nxp) {
     // The level profiles (rho_h, dd, a, c) in LDS, 4 nz doubles: read from
     // global memory in the level loops they are vector loads (the stores
     // keep the compiler from the scalar cache), and each wait for one would
@@ -459,13 +368,13 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
     const int ky = (int)(col / nxp);
     if (kx >= nxh) return;
     if (kx == 0 && ky == 0) {
-        for (int k = 0; k < nz; ++k) spec[(long long)k * ncol + col] = C2<float>{0.0f, 0.0f};
+        for (int k = 0; k < nz; ++k) spec[(long long)k * ncol + col] = C2<TS>{TS(0), TS(0)};
         return;
     }
     const double kk = kx2[kx] + ky2[ky];
     auto at = [&](int k) { return spec + (long long)k * ncol + col; };
     // one forward step: the carry of level k - 1 to level k
-    auto fstep = [&](int k, C2<float> z, double &sc, double &xr, double &xi) {
+    auto fstep = [&](int k, C2<TS> z, double &sc, double &xr, double &xi) {
         const double b = lrho[k] * kk - ldd[k];
         if (k == 0) {
             const double ib = rcp64(b);
@@ -482,7 +391,7 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
     // forward sweep: right-hand sides S levels ahead, loaded at every level
     // (clamped), a carry stored at the end of every whole segment
     double sc = 0.0, xr = 0.0, xi = 0.0;
-    C2<float> zq[S];
+    C2<TS> zq[S];
 #pragma unroll
     for (int u = 0; u < S; ++u) zq[u] = *at(min(u, nz - 1));
     int k0 = 0;
@@ -490,7 +399,7 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
 #pragma unroll
         for (int u = 0; u < S; ++u) {
             const int k = k0 + u;
-            const C2<float> z = zq[u];
+            const C2<TS> z = zq[u];
             zq[u] = *at(min(k + S, nz - 1));
             fstep(k, z, sc, xr, xi);
         }
@@ -503,7 +412,7 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
     for (int u = 0; u < S; ++u)
         if (k0 + u < nz) fstep(k0 + u, zq[u], sc, xr, xi);
     // the top level's solution is its forward value
-    *at(nz - 1) = C2<float>{(float)xr, (float)xi};
+    *at(nz - 1) = C2<TS>{(TS)xr, (TS)xi};
     // back substitution, segment by segment from the top; (xr, xi) carries
     // the solution of the level above, in FP64.  Segment sg needs the carry
     // of segment sg - 1 (none for sg = 0, whose first step starts afresh);
@@ -511,9 +420,9 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
     // single segment no carry is read.
     const int nseg = (nz + S - 1) / S;
     const int top = nz - 2 < 0 ? 0 : nz - 2;  // the highest level still holding its right-hand side
-    C2<float> zs[S];
+    C2<TS> zs[S];
     double csc = 0.0, cxr = 0.0, cxi = 0.0;
-    auto load_seg = [&](int sg, C2<float> *z, double &lsc, double &lxr, double &lxi) {
+    auto load_seg = [&](int sg, C2<TS> *z, double &lsc, double &lxr, double &lxi) {
 #pragma unroll
         for (int u = 0; u < S; ++u) z[u] = *at(min(sg * S + u, top));
         if (nseg > 1) {
@@ -526,17 +435,17 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
     load_seg(nseg - 1, zs, csc, cxr, cxi);
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int lo = sg * S;
-        C2<float> zn[S];
+        C2<TS> zn[S];
         double nsc = 0.0, nxr = 0.0, nxi = 0.0;
         load_seg(max(sg - 1, 0), zn, nsc, nxr, nxi);
-        float sT[S], rT[S], iT[S];
+        TS sT[S], rT[S], iT[S];
 #pragma unroll
         for (int u = 0; u < S; ++u) {
             if (lo + u < nz - 1) {
                 fstep(lo + u, zs[u], csc, cxr, cxi);
-                sT[u] = (float)csc;
-                rT[u] = (float)cxr;
-                iT[u] = (float)cxi;
+                sT[u] = (TS)csc;
+                rT[u] = (TS)cxr;
+                iT[u] = (TS)cxi;
             }
         }
 #pragma unroll
@@ -545,7 +454,7 @@ __global__ __launch_bounds__(256, 3) void thomas_ckpt_kernel(Geo g, C2<float> *s
             if (k < nz - 1) {
                 xr = (double)rT[u] - (double)sT[u] * xr;
                 xi = (double)iT[u] - (double)sT[u] * xi;
-                *at(k) = C2<float>{(float)xr, (float)xi};
+                *at(k) = C2<TS>{(TS)xr, (TS)xi};
             }
         }
 #pragma unroll
@@ -814,8 +723,10 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
         }                                                   \
     } while (0)
     ALLOC(P->spec, nspec * cs);
-    ALLOC(P->xs, nspec * cs);
-    ALLOC(P->cs, nspec * (cs / 2));
+    {
+        const int seg = precision == PYCLES_F64 ? thomas_seg<double>() : thomas_seg<float>();
+        ALLOC(P->ck, 3 * sizeof(double) * (size_t)(g.nz / seg + 1) * g.ny * P->nxp);
+    }
     ALLOC(P->twx, (size_t)g.nx * cs);
     ALLOC(P->twy, (size_t)g.ny * cs);
     ALLOC(P->kx2, P->nxh * 8);
@@ -856,7 +767,7 @@ extern "C" int pycles_pressure_plan_create(const pycles_dims *dims, const double
 
 extern "C" void pycles_pressure_plan_destroy(pycles_pressure_plan *P) {
     if (!P) return;
-    void *ptrs[] = {P->spec, P->xs, P->cs, P->twx, P->twy, P->kx2, P->ky2, P->rho_h,
+    void *ptrs[] = {P->spec, P->ck, P->twx, P->twy, P->kx2, P->ky2, P->rho_h,
                     P->dd, P->a, P->c, P->mean, P->partial, P->cfl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -886,15 +797,9 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->nxp, P->gx, P->loggx, 0);
     PYCLES_CHECK_LAUNCH();
     const long long ncol = (long long)g.ny * P->nxp;
-    if constexpr (sizeof(TS) == 4)
-        hipLaunchKernelGGL(thomas_ckpt_kernel<16>, dim3((unsigned)((ncol + 255) / 256)), dim3(256),
-                           4 * sizeof(double) * (size_t)g.nz, st, g,
-                           (C2<float> *)P->spec, (double *)P->xs, P->kx2, P->ky2, P->rho_h, P->dd, P->a, P->c,
-                           P->nxh, P->nxp);
-    else
-        hipLaunchKernelGGL(thomas_kernel<TS>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, g,
-                           (C2<TS> *)P->spec, (TS *)P->xs, (TS *)P->cs, P->kx2, P->ky2, P->rho_h, P->dd, P->a,
-                           P->c, P->nxh, P->nxp);
+    hipLaunchKernelGGL((thomas_kernel<TS, thomas_seg<TS>()>), dim3((unsigned)((ncol + 255) / 256)), dim3(256),
+                       4 * sizeof(double) * (size_t)g.nz, st, g, (C2<TS> *)P->spec, P->ck, P->kx2, P->ky2,
+                       P->rho_h, P->dd, P->a, P->c, P->nxh, P->nxp);
     PYCLES_CHECK_LAUNCH();
     hipLaunchKernelGGL(ffty_kernel<TS>, ygrid, dim3(by), ysmem, st, g, (C2<TS> *)P->spec,
                        (const C2<TS> *)P->twy, P->logy, P->nxh, P->nxp, P->gx, P->loggx, 1);
