@@ -91,3 +91,8 @@ def test_pressure_plan_rejects_unsupported_sizes(lib):
     plan = C.c_void_p()
     rc = lib.pycles_pressure_plan_create(C.byref(_dims(n=(12, 16, 8))), rho, rho, 0, C.byref(plan))
     assert rc != 0 and not plan.value
+    # the z solve holds its level profiles in LDS: at most 2048 levels
+    lib.pycles_last_error.restype = C.c_char_p
+    rc = lib.pycles_pressure_plan_create(C.byref(_dims(n=(16, 16, 2049))), rho, rho, 0, C.byref(plan))
+    assert rc == -3 and not plan.value
+    assert b"2048" in lib.pycles_last_error()
