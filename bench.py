@@ -59,6 +59,9 @@ KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, 
 # packed FP32, so half of it is their ceiling).
 KERNEL_FLOPS = {"adv_x": 755, "adv_y": 755, "adv_z": 755, "adv_z_rk3": 775}
 VALU_PEAK_TFLOPS = 157.3
+# Field passes per moist SSP-RK3 step (thermo 6, scalar advection 7,
+# momentum advection 7, RK update 20, projection 22 per stage; x3)
+STEP_PASSES = 186
 KERNEL_NAMES = {"thermo": "thermo_sa_f32_kernel<true,true>", "adv_x": "adv::adv_x<PolicyF32,5,5,2,true>",
                 "adv_y": "adv::adv_y<PolicyF32,5,5,2,true>", "adv_z": "adv::adv_z<PolicyF32,5,5,2,true>",
                 "rk3": "rk3_kernel<float,...>", "adv_z_rk3": "adv::adv_z<PolicyF32,5,5,2,true> (RK3 fused)"}
@@ -141,24 +144,66 @@ def field_errors(sim32, sim64):
     return out
 
 
-def error_study(nl, Gr, RS, Th, fields, learned, nsteps):
-    """FP32 + learned against FP64 from the same state after 1 and nsteps steps."""
-    import torch
-    a = make_sim(nl, Gr, RS, Th, fields, "f32", learned)
-    b = make_sim(nl, Gr, RS, Th, fields, "f64")
+def perturbed_fields(fields, kind):
+    """The initial state with a perturbation whose growth sets the floor of any
+    comparison with the FP64 run:
+      ulp64  every nonzero prognostic value moved up by one FP64 ulp;
+      fp32   the state exactly as the FP32 form stores it (u, v, w rounded to
+             FP32; s, qt as level mean + FP32-rounded deviation), kept in FP64."""
     out = {}
-    for sim in (a, b):
-        sim.Th.mlp = learned if sim is a else None
+    for n, f in fields.items():
+        if kind == "ulp64":
+            out[n] = np.where(f != 0.0, np.nextafter(f, np.inf), 0.0)
+        elif n in ("s", "qt"):
+            base = f.mean(axis=(1, 2), keepdims=True)
+            out[n] = base + (f - base).astype(np.float32).astype(np.float64)
+        else:
+            out[n] = f.astype(np.float32).astype(np.float64)
+    return out
+
+
+def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True):
+    """FP32 + learned against FP64 from the same state after 1 and nsteps
+    steps; with `floors`, also FP64 runs from the perturbed states of
+    `perturbed_fields` against the same FP64 run (their noise floors)."""
+    import torch
+    runs = {"fp32_learned": make_sim(nl, Gr, RS, Th, fields, "f32", learned),
+            "fp64": make_sim(nl, Gr, RS, Th, fields, "f64")}
+    if floors:
+        for kind in ("ulp64", "fp32"):
+            runs["fp64_from_" + kind] = make_sim(nl, Gr, RS, Th, perturbed_fields(fields, kind), "f64")
+    out = {"errors_vs_fp64": {}, "noise_floor_fp64": {}}
+    for name, sim in runs.items():
+        sim.Th.mlp = learned if name == "fp32_learned" else None
         sim.begin_run()
     for n in range(1, nsteps + 1):
-        for sim in (a, b):
-            sim.Th.mlp = learned if sim is a else None
+        for name, sim in runs.items():
+            sim.Th.mlp = learned if name == "fp32_learned" else None
             sim.step()
         if n == 1 or n == nsteps:
             torch.cuda.synchronize()
-            out[f"after_{n}_steps"] = field_errors(a, b)
-            log(f"[bench] errors after {n} steps: T {out[f'after_{n}_steps']['temperature']}")
-    a.Th.mlp = None
+            key = f"after_{n}_steps"
+            out["errors_vs_fp64"][key] = field_errors(runs["fp32_learned"], runs["fp64"])
+            for name in runs:
+                if name.startswith("fp64_from_"):
+                    out["noise_floor_fp64"].setdefault(name, {})[key] = field_errors(runs[name], runs["fp64"])
+            e = out["errors_vs_fp64"][key]
+            log(f"[bench] errors after {n} steps: T {e['temperature']} qt {e['qt']}")
+            for name, fl in out["noise_floor_fp64"].items():
+                log(f"[bench] floor {name} after {n} steps: T {fl[key]['temperature']} qt {fl[key]['qt']}")
+    runs["fp32_learned"].Th.mlp = None
+    return out
+
+
+def error_gate(errors, tol=1e-5):
+    """The metric's gate: T and qt within tol (max relative) of FP64."""
+    out = {"tolerance": tol, "fields": ["temperature", "qt"], "failing": []}
+    for key, e in errors.items():
+        for f in out["fields"]:
+            v = e[f]["max_rel"]
+            if not v <= tol:
+                out["failing"].append(f"{f} {key}: {v:.3g}")
+    out["pass"] = not out["failing"]
     return out
 
 
@@ -180,6 +225,16 @@ def train_guess(Th, pairs_log2=22, epochs=40):
             "heldout_rms_K": float(np.sqrt(np.mean(err ** 2))), "heldout_max_K": float(np.abs(err).max()),
             "inference": "bf16 MFMA 32x32x16, FP32 accumulate, fused into the thermodynamics kernel"}
     return g, info
+
+
+def load_guess(Th, path):
+    from pycles_amd.LearnedGuess import SaturationGuess
+    with np.load(path, allow_pickle=False) as z:
+        st = {"params": z["params"], "norm": z["norm"]}
+    g = SaturationGuess(Th.CC)
+    g.load_state_dict(st)
+    return g, {"loaded_from": os.path.basename(path),
+               "inference": "bf16 MFMA 32x32x16, FP32 accumulate, fused into the thermodynamics kernel"}
 
 
 def per_kernel(phases, steps):
@@ -229,6 +284,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--guess-pairs-log2", type=int, default=22)
     ap.add_argument("--guess-epochs", type=int, default=40)
+    ap.add_argument("--no-floor", action="store_true", help="skip the FP64 noise-floor runs of the error study")
+    ap.add_argument("--save-guess", default=None, help="write the trained network (npz) here")
+    ap.add_argument("--load-guess", default=None, help="use this trained network (npz) instead of training")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -251,9 +309,16 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copy calibration: 1 GiB read + 1 GiB written")
         del a, b
-    log("[bench] training the saturation guess")
-    guess, ginfo = train_guess(Th, args.guess_pairs_log2, args.guess_epochs)
-    log(f"[bench] trained in {ginfo['train_s']} s, held-out rms {ginfo['heldout_rms_K']:.4f} K")
+    if args.load_guess:
+        guess, ginfo = load_guess(Th, args.load_guess)
+        log(f"[bench] loaded the saturation guess from {args.load_guess}")
+    else:
+        log("[bench] training the saturation guess")
+        guess, ginfo = train_guess(Th, args.guess_pairs_log2, args.guess_epochs)
+        log(f"[bench] trained in {ginfo['train_s']} s, held-out rms {ginfo['heldout_rms_K']:.4f} K")
+    if args.save_guess:
+        st = guess.state_dict()
+        np.savez(args.save_guess, params=st["params"], norm=st["norm"])
 
     log("[bench] FP32 + learned form")
     simL, msL, phL, cntL = run_form(nl, Gr, RS, Th, fields, "f32", args.steps, args.warmup, learned=guess)
@@ -296,6 +361,12 @@ def main():
             "unit": "GB/s", "frac": kd["hbm_frac"],
             "traffic": tr["bytes_per_launch"] if tr else None,
             "algorithmic_bytes_per_launch": kd["bytes_per_cell"] * ncell, "ms_per_launch": kd["ms_per_launch"]}
+    # the whole step against its algorithmic bytes: 186 field passes per
+    # moist SSP-RK3 step (62 per stage), FP32 storage
+    step_bytes = STEP_PASSES * 4 * ncell
+    step_roof = {"passes_per_step": STEP_PASSES, "bytes_per_step": step_bytes,
+                 "achieved_GBs": round(step_bytes / (msL * 1e-3) / 1e9, 1),
+                 "frac": round(step_bytes / (msL * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     ginfo["newton_steps"] = Th.newton_steps
     ginfo["saturated_fraction"] = round(cntL[0] / (ncell * 3 * args.steps), 4)
     ginfo["newton_iterations_per_saturated_cell"] = round(cntL[1] / max(cntL[0], 1), 3)
@@ -309,7 +380,7 @@ def main():
                                   f"{c['order'][1]}, FFT+tridiagonal pressure, dt={c['dt']} s, "
                                   "FP32 + learned saturation guess", "cells": ncell},
            "phases_ms_per_step": {k: round(v / args.steps, 4) for k, v in phL.items()},
-           "roofline": roof, "kernels": kernels, "learned_guess": ginfo}
+           "roofline": roof, "step_roofline": step_roof, "kernels": kernels, "learned_guess": ginfo}
     if out64:
         out["fp64"] = out64
         out["speedup_vs_fp64"] = round(out64["ms_per_step"] / msL, 3)
@@ -317,7 +388,11 @@ def main():
         out["fp32_plain"] = out_plain
     if not args.no_fp64 and args.error_steps > 0:
         log(f"[bench] error study: {args.error_steps} steps")
-        out["errors_vs_fp64"] = error_study(nl, Gr, RS, Th, fields, guess, args.error_steps)
+        es = error_study(nl, Gr, RS, Th, fields, guess, args.error_steps, floors=not args.no_floor)
+        out["errors_vs_fp64"] = es["errors_vs_fp64"]
+        out["error_gate"] = error_gate(es["errors_vs_fp64"])
+        if es["noise_floor_fp64"]:
+            out["noise_floor_fp64"] = es["noise_floor_fp64"]
         torch.cuda.empty_cache()
     if not args.no_cpu:
         log("[bench] CPU baseline")

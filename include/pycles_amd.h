@@ -177,9 +177,17 @@ int pycles_apply_z_bcs(const pycles_dims *dims, int nv, void *const *values,
 int pycles_level_mean(const pycles_dims *dims, const void *field, double *mean, int precision,
                       void *stream);
 /* compute_cfl_max (TimeStepping.pyx:222-263) into one FP64 device value
- * (max over interior of dt*(|u|/dx+|v|/dy+|w|/dz), plus 1e-11). */
+ * (max over interior of dt*(|u|/dx+|v|/dy+|w|/dz), plus 1e-11).  A NaN or
+ * infinite velocity makes the value NaN (the reference's fmax would drop a
+ * NaN). */
 int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
                    double dt, double *out, int precision, void *stream);
+/* Waits for the stream and copies a device CFL value (pycles_cfl_max or the
+ * cfl_out of pycles_pressure_update) to *host_cfl: the host read of
+ * TimeStepping.adjust_timestep (TimeStepping.pyx:94-98, 255-266).  Returns
+ * PYCLES_ERR_NONFINITE when the value is NaN or infinite, i.e. when a
+ * velocity is not finite (the reference lets NaNs run on silently). */
+int pycles_cfl_read(const double *dev_cfl, double *host_cfl, void *stream);
 
 /* ---- pressure projection ----------------------------------------------- */
 typedef struct pycles_pressure_plan pycles_pressure_plan;
@@ -204,14 +212,27 @@ int pycles_divergence(const pycles_dims *dims, const pycles_ref *ref, const void
                       const void *v, const void *w, void *div, int precision, void *stream);
 
 /* ---- layout conversion ------------------------------------------------- */
-/* Reference ghosted [i][j][k] (nlg = n + 2gw in every direction, FP64) to
- * device layout in the given precision, minus base (may be NULL). */
+/* The entry points a caller holding the reference's arrays uses at ingress
+ * and egress (the reference indexes them as Csrc/grid.h:2-24 lays out a
+ * variable of PV.values/DV.values, PrognosticVariables.pyx:77-79).
+ * Reference ghosted [i][j][k] (nlg = n + 2gw in every direction, FP64) to
+ * device layout in the given precision, minus base (may be NULL); the x/y
+ * ghost columns of ref_layout are not read.  Both arrays are device arrays. */
 int pycles_layout_ingress(const pycles_dims *dims, const double *ref_layout, void *dev,
                           const double *base, int precision, void *stream);
 /* Device layout to the reference layout (x/y ghosts = periodic images),
  * plus base (may be NULL). */
 int pycles_layout_egress(const pycles_dims *dims, const void *dev, double *ref_layout,
                          const double *base, int precision, void *stream);
+
+/* ---- launch geometry (diagnostic) -------------------------------------- */
+/* z chunks of the z-chunked kernels for dims: kernel 0 = pressure
+ * correction (chunks of the interior levels [gw, nz+gw), the last one taking
+ * the remainder: chunk b is [gw + b*len, b+1 == n ? nz+gw : gw + (b+1)*len)),
+ * 1 = advection y pass (rows [0, ny), chunk b is [b*len, min((b+1)*len, ny))),
+ * 2 = advection z pass (interior levels, as 1 from gw), 3 = thermodynamics
+ * (all nzg levels, as 1).  No reference counterpart. */
+int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len);
 
 /* ---- learned first guess of the saturation adjustment -----------------
  * No counterpart in the reference: its eos_c (thermodynamics_sa.h:17-66)

@@ -422,6 +422,82 @@ double orc_entropy_sa(double p0, double T, double qt, double ql, double qi) {
     return sd_c(pd, T) * (1.0 - qt) + sv_c(pv, T) * qt + sc_c(L, T) * (ql + qi);
 }
 
+/* buoyancy_update_sa, thermodynamics_sa.h:98-132 */
+void orc_buoyancy_update_sa(const orc_dims *d, const double *alpha0_half, const double *alpha,
+                            double *buoyancy, double *wt) {
+    long st[3];
+    strides(d, st);
+    const long n0 = d->nlg[0], n1 = d->nlg[1], n2 = d->nlg[2];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 0; k < n2 - 1; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                buoyancy[ijk] = buoyancy_c(alpha0_half[k], alpha[ijk]);
+            }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 1; k < n2 - 3; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                wt[ijk] = wt[ijk] + orc_interp_2(buoyancy[ijk], buoyancy[ijk + 1]);
+            }
+}
+
+/* bvf_sa, thermodynamics_sa.h:134-182 */
+void orc_bvf_sa(const orc_dims *d, const orc_lookup *lt, const double *p0, const double *T,
+                const double *qt, const double *qv, double *theta_rho, double *bvf) {
+    long st[3];
+    strides(d, st);
+    const long n0 = d->nlg[0], n1 = d->nlg[1], n2 = d->nlg[2];
+    const double dzi = 1.0 / d->dx[2];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 1; k < n2 - 1; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                theta_rho[ijk] = theta_rho_c(p0[k], T[ijk], qt[ijk], qv[ijk]);
+            }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 2; k < n2 - 2; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                if (qv[ijk] < qt[ijk]) {
+                    const double Tk = T[ijk];
+                    const double Lv = orc_latent_heat_variable(Tk);
+                    const double pv_star = orc_lookup_eval(lt, Tk);
+                    const double rsl = C_EPS_V * pv_star / (p0[k] - pv_star);
+                    const double gamma_w =
+                        C_G / C_CPD * (1.0 / (1.0 - qt[ijk])) * (1.0 + Lv * rsl / (C_RD * Tk)) /
+                        (cpm_c(qt[ijk]) / C_CPD + Lv * Lv * (C_EPS_V + rsl) * rsl / (C_CPD * C_RD * Tk * Tk));
+                    const double dTdz = (orc_interp_2(T[ijk], T[ijk + 1]) - orc_interp_2(T[ijk - 1], T[ijk])) * dzi;
+                    const double dqtdz = (orc_interp_2(qt[ijk], qt[ijk + 1]) - orc_interp_2(qt[ijk - 1], qt[ijk])) * dzi;
+                    bvf[ijk] = C_G / Tk * (dTdz + gamma_w) * (1.0 + Lv * rsl / (C_RD * Tk)) - dqtdz / (1.0 - qt[ijk]);
+                } else {
+                    bvf[ijk] = C_G / theta_rho[ijk] *
+                               (orc_interp_2(theta_rho[ijk], theta_rho[ijk + 1]) -
+                                orc_interp_2(theta_rho[ijk - 1], theta_rho[ijk])) * dzi;
+                }
+            }
+}
+
+/* thetali_update, thermodynamics_sa.h:185-211 */
+void orc_thetali_update(const orc_dims *d, const double *p0, const double *T, const double *qt,
+                        const double *ql, const double *qi, double *thetali) {
+    long st[3];
+    strides(d, st);
+    const long n0 = d->nlg[0], n1 = d->nlg[1], n2 = d->nlg[2];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 0; k < n2; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                const double Lv = orc_latent_heat_variable(T[ijk]);
+                thetali[ijk] = thetali_c(p0[k], T[ijk], qt[ijk], ql[ijk], qi[ijk], Lv);
+            }
+}
+
 /* ThermodynamicsSA.update, ThermodynamicsSA.pyx:176-213 */
 void orc_thermo_sa_update(const orc_dims *d, const orc_ref *rs, const orc_lookup *lt,
                           double *s, double *qt, double *wt, double *T, double *qv, double *ql,
@@ -460,61 +536,9 @@ void orc_thermo_sa_update(const orc_dims *d, const orc_ref *rs, const orc_lookup
         stats->max_iterations = n_max;
         stats->capped = n_cap;
     }
-    /* buoyancy_update_sa, thermodynamics_sa.h:98-132 */
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 0; k < n2 - 1; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                buoyancy[ijk] = buoyancy_c(rs->alpha0_half[k], alpha[ijk]);
-            }
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 1; k < n2 - 3; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                wt[ijk] = wt[ijk] + orc_interp_2(buoyancy[ijk], buoyancy[ijk + 1]);
-            }
-    /* bvf_sa, thermodynamics_sa.h:134-182 */
-    const double dzi = 1.0 / d->dx[2];
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 1; k < n2 - 1; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                theta_rho[ijk] = theta_rho_c(p0[k], T[ijk], qt[ijk], qv[ijk]);
-            }
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 2; k < n2 - 2; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                if (qv[ijk] < qt[ijk]) {
-                    const double Tk = T[ijk];
-                    const double Lv = orc_latent_heat_variable(Tk);
-                    const double pv_star = orc_lookup_eval(lt, Tk);
-                    const double rsl = C_EPS_V * pv_star / (p0[k] - pv_star);
-                    const double gamma_w =
-                        C_G / C_CPD * (1.0 / (1.0 - qt[ijk])) * (1.0 + Lv * rsl / (C_RD * Tk)) /
-                        (cpm_c(qt[ijk]) / C_CPD + Lv * Lv * (C_EPS_V + rsl) * rsl / (C_CPD * C_RD * Tk * Tk));
-                    const double dTdz = (orc_interp_2(T[ijk], T[ijk + 1]) - orc_interp_2(T[ijk - 1], T[ijk])) * dzi;
-                    const double dqtdz = (orc_interp_2(qt[ijk], qt[ijk + 1]) - orc_interp_2(qt[ijk - 1], qt[ijk])) * dzi;
-                    bvf[ijk] = C_G / Tk * (dTdz + gamma_w) * (1.0 + Lv * rsl / (C_RD * Tk)) - dqtdz / (1.0 - qt[ijk]);
-                } else {
-                    bvf[ijk] = C_G / theta_rho[ijk] *
-                               (orc_interp_2(theta_rho[ijk], theta_rho[ijk + 1]) -
-                                orc_interp_2(theta_rho[ijk - 1], theta_rho[ijk])) * dzi;
-                }
-            }
-    /* thetali_update, thermodynamics_sa.h:185-211 */
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 0; k < n2; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                const double Lv = orc_latent_heat_variable(T[ijk]);
-                thetali[ijk] = thetali_c(p0[k], T[ijk], qt[ijk], ql[ijk], qi[ijk], Lv);
-            }
+    orc_buoyancy_update_sa(d, rs->alpha0_half, alpha, buoyancy, wt);
+    orc_bvf_sa(d, lt, p0, T, qt, qv, theta_rho, bvf);
+    orc_thetali_update(d, p0, T, qt, ql, qi, thetali);
 }
 
 /* thermodynamics_dry.h:8-10 */
@@ -523,6 +547,51 @@ double orc_eos_dry(double pd, double s) {
 }
 /* ThermodynamicsDry.entropy -> sd_c (ThermodynamicsDry.pyx:93-97) */
 double orc_entropy_dry(double pd, double T) { return sd_c(pd, T); }
+
+/* buoyancy_update, thermodynamics_dry.h:39-73 */
+void orc_buoyancy_update_dry(const orc_dims *d, const double *alpha0_half, const double *alpha,
+                             double *buoyancy, double *wt) {
+    long st[3];
+    strides(d, st);
+    const long n0 = d->nlg[0], n1 = d->nlg[1], n2 = d->nlg[2];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0 - 1; ++i)
+        for (long j = 0; j < n1 - 1; ++j)
+            for (long k = 0; k < n2 - 1; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                buoyancy[ijk] = buoyancy_c(alpha0_half[k], alpha[ijk]);
+            }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0 - 1; ++i)
+        for (long j = 0; j < n1 - 1; ++j)
+            for (long k = 0; k < n2 - 1; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                wt[ijk] += orc_interp_2(buoyancy[ijk], buoyancy[ijk + 1]);
+            }
+}
+
+/* bvf_dry, thermodynamics_dry.h:75-110 */
+void orc_bvf_dry(const orc_dims *d, const double *p0, const double *T, double *theta, double *bvf) {
+    long st[3];
+    strides(d, st);
+    const long n0 = d->nlg[0], n1 = d->nlg[1], n2 = d->nlg[2];
+    const double dzi = 1.0 / d->dx[2];
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 1; k < n2; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                theta[ijk] = T[ijk] / orc_exner(p0[k]);
+            }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j)
+            for (long k = 2; k < n2 - 1; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                bvf[ijk] = C_G / theta[ijk] *
+                           (orc_interp_2(theta[ijk], theta[ijk + 1]) - orc_interp_2(theta[ijk - 1], theta[ijk])) * dzi;
+            }
+}
 
 /* ThermodynamicsDry.update, ThermodynamicsDry.pyx:105-120 */
 void orc_thermo_dry_update(const orc_dims *d, const orc_ref *rs, double *s, double *wt,
@@ -541,38 +610,8 @@ void orc_thermo_dry_update(const orc_dims *d, const orc_ref *rs, double *s, doub
                 T[ijk] = orc_eos_dry(p0[k], s[ijk]);
                 alpha[ijk] = orc_alpha(p0[k], T[ijk], 0.0, 0.0);
             }
-    /* buoyancy_update, thermodynamics_dry.h:39-73 */
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0 - 1; ++i)
-        for (long j = 0; j < n1 - 1; ++j)
-            for (long k = 0; k < n2 - 1; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                buoyancy[ijk] = buoyancy_c(rs->alpha0_half[k], alpha[ijk]);
-            }
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0 - 1; ++i)
-        for (long j = 0; j < n1 - 1; ++j)
-            for (long k = 0; k < n2 - 1; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                wt[ijk] += orc_interp_2(buoyancy[ijk], buoyancy[ijk + 1]);
-            }
-    /* bvf_dry, thermodynamics_dry.h:75-110 */
-    const double dzi = 1.0 / d->dx[2];
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 1; k < n2; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                theta[ijk] = T[ijk] / orc_exner(p0[k]);
-            }
-#pragma omp parallel for schedule(static)
-    for (long i = 0; i < n0; ++i)
-        for (long j = 0; j < n1; ++j)
-            for (long k = 2; k < n2 - 1; ++k) {
-                const long ijk = i * st[0] + j * st[1] + k;
-                bvf[ijk] = C_G / theta[ijk] *
-                           (orc_interp_2(theta[ijk], theta[ijk + 1]) - orc_interp_2(theta[ijk - 1], theta[ijk])) * dzi;
-            }
+    orc_buoyancy_update_dry(d, rs->alpha0_half, alpha, buoyancy, wt);
+    orc_bvf_dry(d, p0, T, theta, bvf);
 }
 
 /* ------------------------------------------------------------------------ */

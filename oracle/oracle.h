@@ -104,6 +104,18 @@ void orc_thermo_sa_update(const orc_dims *d, const orc_ref *rs, const orc_lookup
                           double *qi, double *alpha, double *buoyancy, double *theta_rho,
                           double *bvf, double *thetali, int do_clip, int max_iter,
                           orc_sa_stats *stats);
+/* The parts of ThermodynamicsSA.update (thermodynamics_sa.h:98-211) and
+ * ThermodynamicsDry.update (thermodynamics_dry.h:39-110), over the ghosted
+ * reference layout. */
+void orc_buoyancy_update_sa(const orc_dims *d, const double *alpha0_half, const double *alpha,
+                            double *buoyancy, double *wt);
+void orc_bvf_sa(const orc_dims *d, const orc_lookup *lt, const double *p0, const double *T,
+                const double *qt, const double *qv, double *theta_rho, double *bvf);
+void orc_thetali_update(const orc_dims *d, const double *p0, const double *T, const double *qt,
+                        const double *ql, const double *qi, double *thetali);
+void orc_buoyancy_update_dry(const orc_dims *d, const double *alpha0_half, const double *alpha,
+                             double *buoyancy, double *wt);
+void orc_bvf_dry(const orc_dims *d, const double *p0, const double *T, double *theta, double *bvf);
 double orc_eos_dry(double pd, double s);
 double orc_entropy_dry(double pd, double T);
 /* ThermodynamicsDry.update (ThermodynamicsDry.pyx:105-120). */

@@ -67,10 +67,14 @@ class TimeStepping:
             _capi.cfl_max(D.dims_of(Gr), PV.ptr(n[0]), PV.ptr(n[1]), PV.ptr(n[2]), self.dt,
                           self.cfl_dev.data_ptr(), PV.precision, D.stream())
         self._cfl_ready = False
-        self.cfl_max = float(self.cfl_dev.item())
-        # the reference stops on a negative maximum; a NaN velocity makes the
-        # device maximum NaN (its bits exceed every finite value's), and that
-        # stops the run too instead of setting a NaN time step
+        # the library's read raises (PYCLES_ERR_NONFINITE) when a velocity is
+        # NaN or infinite, which the device maximum carries, instead of
+        # setting a NaN time step; the reference stops on a negative maximum
+        self.accept_cfl(_capi.cfl_read(self.cfl_dev.data_ptr(), D.stream()))
+
+    def accept_cfl(self, value):
+        """Take the step's CFL maximum (TimeStepping.pyx:255-263)."""
+        self.cfl_max = float(value)
         if not self.cfl_max >= 0.0:
             raise RuntimeError(f"CFL_MAX = {self.cfl_max} killing simulation!")
 

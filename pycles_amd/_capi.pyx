@@ -82,6 +82,8 @@ cdef extern from "pycles_amd.h":
                           void *stream)
     int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
                        double dt, double *out, int precision, void *stream)
+    int pycles_cfl_read(const double *dev_cfl, double *host_cfl, void *stream)
+    int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len)
     int pycles_pressure_plan_create(const pycles_dims *dims, const double *rho0_host,
                                     const double *rho0_half_host, int precision,
                                     pycles_pressure_plan **plan)
@@ -319,6 +321,20 @@ def level_mean(Dims dims, field, mean, int precision, stream):
 
 def cfl_max(Dims dims, u, v, w, double dt, out, int precision, stream):
     _check(pycles_cfl_max(&dims.d, _p(u), _p(v), _p(w), dt, <double *> _p(out), precision, _p(stream)))
+
+
+def cfl_read(dev_cfl, stream):
+    """Synchronise the stream and return the device CFL value; raises
+    PyclesError (status PYCLES_ERR_NONFINITE) if it is NaN or infinite."""
+    cdef double h = 0.0
+    _check(pycles_cfl_read(<const double *> _p(dev_cfl), &h, _p(stream)))
+    return h
+
+
+def launch_chunks(int kernel, Dims dims):
+    cdef int nch = 0, ln = 0
+    _check(pycles_launch_chunks(kernel, &dims.d, &nch, &ln))
+    return nch, ln
 
 
 def divergence(Dims dims, Ref ref, u, v, w, div, int precision, stream):

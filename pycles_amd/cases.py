@@ -84,7 +84,7 @@ CONFIGS = {
     "moist512": dict(nx=512, ny=512, nz=256, gw=3, dx=50.0, dy=50.0, dz=20.0, moist=True,
                      order=(5, 5), dt=1.0, steps=100, seed=3, sigma=0.3, cloud=(500.0, 1500.0)),
     "moist1024": dict(nx=1024, ny=1024, nz=192, gw=5, dx=25.0, dy=25.0, dz=10.0, moist=True,
-                      order=(9, 5), dt=0.5, steps=20, seed=5, sigma=1.0, cloud=(500.0, 1500.0)),
+                      order=(9, 5), dt=0.5, steps=20, seed=5, sigma=1.0, cloud=(500.0, 880.0)),
 }
 
 

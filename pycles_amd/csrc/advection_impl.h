@@ -782,19 +782,15 @@ int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
         PYCLES_CHECK_LAUNCH();
     }
     // y: split the march into chunks so the grid holds enough waves.
-    const long long ycols = (long long)((g.nx + 63) / 64) * ((g.nz + 3) / 4);
-    int ych = 1;
-    while (ycols * ych < 2048 && g.ny / (ych * 2) >= 32) ych *= 2;
-    const int jchunk = (g.ny + ych - 1) / ych;
+    const ZSplit ys = advy_zsplit(g.nx, g.ny, g.nz);
+    const int ych = ys.nch, jchunk = ys.len;
     if (fl.passes & 2) {
         hipLaunchKernelGGL((adv_y<P, SO, MO, NSC, MOM>), dim3((g.nx + 63) / 64, (g.nz + 3) / 4, ych),
                            dim3(64, 4), 0, st, g, pr, fl, jchunk);
         PYCLES_CHECK_LAUNCH();
     }
-    const long long zcols = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
-    int zch = 1;
-    while (zcols * zch < 2048 && g.nz / (zch * 2) >= 32) zch *= 2;
-    const int kchunk = (g.nz + zch - 1) / zch;
+    const ZSplit zs = advz_zsplit(g.nx, g.ny, g.nz);
+    const int zch = zs.nch, kchunk = zs.len;
     if (fl.passes & 4) {
         const dim3 zg((g.nx + 63) / 64, (g.ny + 3) / 4, zch);
         if (fl.rk < 0)

@@ -54,6 +54,78 @@ __device__ __forceinline__ long long gidx(const Geo &g, int i, int j, int k) {
     return ((long long)k * g.ny + j) * g.nx + i;
 }
 
+// Split of n levels (or rows) into the chunks of a kernel grid's z dimension:
+// nch chunks of len levels each, the last one holding what is left.
+struct ZSplit {
+    int nch, len;
+};
+// Chunks of ceil(n / want) levels, recounted so that none is empty; the last
+// chunk holds n - (nch - 1) * len levels, between 1 and len.
+static inline ZSplit zsplit_ceil(int n, int want) {
+    want = want < 1 ? 1 : (want > n ? n : want);
+    const int len = (n + want - 1) / want;
+    return ZSplit{(n + len - 1) / len, len};
+}
+// Chunks of floor(n / nch) levels with the last one also taking the
+// remainder (n - (nch - 1) * len levels), so that every chunk holds at least
+// minlen levels; a single chunk when n < 2 * minlen.
+static inline ZSplit zsplit_floor(int n, int want, int minlen) {
+    const int cap = minlen > 0 ? n / minlen : n;
+    int nch = want < cap ? want : cap;
+    if (nch < 1) nch = 1;
+    return ZSplit{nch, n / nch};
+}
+// The pressure correction's chunks of the interior levels (~16k waves; the
+// last chunk holds the top gw levels, the first the bottom gw, since the
+// boundary fill of each end reads them).
+static inline ZSplit correct_zsplit(long long plane, int nz, int gw) {
+    int want = 1;
+    while (plane / 64 * want < 16384 && nz / (2 * want) >= 2 * gw && want < 16) want *= 2;
+    return zsplit_floor(nz, want, gw);
+}
+// The advection y pass's chunks of the ny rows and the z pass's chunks of the
+// interior levels (at least 2048 blocks, chunks of at least 32).
+static inline ZSplit advy_zsplit(int nx, int ny, int nz) {
+    const long long cols = (long long)((nx + 63) / 64) * ((nz + 3) / 4);
+    int want = 1;
+    while (cols * want < 2048 && ny / (want * 2) >= 32) want *= 2;
+    return zsplit_ceil(ny, want);
+}
+static inline ZSplit advz_zsplit(int nx, int ny, int nz) {
+    const long long cols = (long long)((nx + 63) / 64) * ((ny + 3) / 4);
+    int want = 1;
+    while (cols * want < 2048 && nz / (want * 2) >= 32) want *= 2;
+    return zsplit_ceil(nz, want);
+}
+// The thermodynamics kernels' chunks of the nzg ghosted levels (~4096 blocks;
+// their LDS holds len + 2 levels, so no chunk is longer than len).
+static inline ZSplit thermo_zsplit(int nx, int ny, int nzg) {
+    const long long cols = (long long)((nx + 63) / 64) * ((ny + 3) / 4);
+    int want = 1;
+    while (cols * want < 4096 && nzg / (want * 2) >= 16) want *= 2;
+    return zsplit_ceil(nzg, want);
+}
+
+// CFL maximum of non-negative values.  A NaN or infinite value makes the
+// result NaN: fmax would drop a NaN, and the host reads the result as the
+// step's check that the velocities are finite.
+__device__ __forceinline__ double cfl_fold(double m, double c) {
+    return (c <= 1.7976931348623157e308 && m == m) ? fmax(m, c) : __longlong_as_double(0x7ff8000000000000LL);
+}
+// Block maximum into *out by an integer atomic max: non-negative doubles
+// order like their bit patterns, and the positive quiet NaN above them all.
+__device__ __forceinline__ void cfl_block_max(double m, double *out) {
+    __shared__ double red[256];
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = cfl_fold(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicMax((unsigned long long *)out, (unsigned long long)__double_as_longlong(red[0]));
+}
+
 // Periodic wrap for offsets smaller than the extent.
 __device__ __forceinline__ int wrapi(int i, int n) {
     return i < 0 ? i + n : (i >= n ? i - n : i);

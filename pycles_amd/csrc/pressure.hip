@@ -533,8 +533,10 @@ __global__ __launch_bounds__(256) void correct_kernel(Geo g, TS *u, TS *v, TS *w
                                                       double dt, double *cfl, int kchunk) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
+    // chunks of kchunk levels; the last one also takes the remainder, so
+    // every chunk holds at least kchunk >= gw levels (or the whole column)
     const int k0 = g.gw + blockIdx.z * kchunk;
-    const int k1 = min(k0 + kchunk, g.nz + g.gw);
+    const int k1 = blockIdx.z + 1 == gridDim.z ? g.nz + g.gw : k0 + kchunk;
     double cm = 0.0;
     if (i < g.nx) {
         const int ip = i + 1 == g.nx ? 0 : i + 1;
@@ -582,13 +584,14 @@ __global__ __launch_bounds__(256) void correct_kernel(Geo g, TS *u, TS *v, TS *w
                 if (cfl) {
                     const double c = dt * (fabs((double)un) * g.dxi[0] + fabs((double)vn) * g.dxi[1] +
                                            fabs((double)wn) * g.dxi[2]);
-                    cm = fmax(cm, c);
+                    cm = cfl_fold(cm, c);
                 }
                 pc = pt;
             }
         }
-        // z ghost levels from the corrected boundary levels this thread wrote
-        // (chunks hold at least 2 gw levels)
+        // z ghost levels from the corrected boundary levels this thread wrote:
+        // the first chunk holds levels gw..2gw-1 and the last nz..nz+gw-1
+        // (each chunk has at least gw levels unless it is the whole column)
         if (k0 == g.gw) {
             for (int m = 0; m < g.gw; ++m) {
                 u[gidx(g, i, j, g.gw - 1 - m)] = u[gidx(g, i, j, g.gw + m)];
@@ -607,16 +610,7 @@ __global__ __launch_bounds__(256) void correct_kernel(Geo g, TS *u, TS *v, TS *w
                 w[gidx(g, i, j, g.nzg - g.gw + m)] = TS(-1.0) * w[gidx(g, i, j, g.nzg - g.gw - m)];
         }
     }
-    if (cfl) {
-        __shared__ double red[256];
-        red[threadIdx.x] = cm;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicMax((unsigned long long *)cfl, (unsigned long long)__double_as_longlong(red[0]));
-    }
+    if (cfl) cfl_block_max(cm, cfl);
 }
 
 __global__ void cfl_zero_kernel(double *c) { *c = 0.0; }
@@ -808,10 +802,9 @@ static int run_pressure(pycles_pressure_plan *P, const pycles_ref *ref, void *u,
                        (const C2<TS> *)P->spec, (TS *)p, (const C2<TS> *)P->twx, P->logx, P->nxh, P->nxp, logS);
     PYCLES_CHECK_LAUNCH();
     if (cfl_out) hipLaunchKernelGGL(cfl_zero_kernel, dim3(1), dim3(1), 0, st, P->cfl);
-    // z chunks (at least 2 gw levels each) so the grid holds ~16k waves
-    int nch = 1;
-    while ((long long)g.plane / 64 * nch < 16384 && g.nz / (2 * nch) >= 2 * g.gw && nch < 16) nch *= 2;
-    const int kchunk = (g.nz + nch - 1) / nch;
+    // z chunks so the grid holds ~16k waves
+    const ZSplit zs = correct_zsplit(g.plane, g.nz, g.gw);
+    const int nch = zs.nch, kchunk = zs.len;
     hipLaunchKernelGGL(correct_kernel<TS>, dim3((g.nx + 255) / 256, g.ny, nch), dim3(256), 0, st, g, (TS *)u,
                        (TS *)v, (TS *)w, (const TS *)p, P->mean, dt, cfl_out ? P->cfl : nullptr, kchunk);
     PYCLES_CHECK_LAUNCH();

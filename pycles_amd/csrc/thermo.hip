@@ -135,13 +135,6 @@ extern int pycles_check_dims(const pycles_dims *d);
 
 int thermo_sa_f32_launch(const Geo &g, const SAArgs &a, dim3 grd, dim3 blk, hipStream_t st);
 
-// Chunks of the column so the grid holds at least ~4096 blocks.
-static int thermo_kchunk(const Geo &g) {
-    const long long cols = (long long)((g.nx + 63) / 64) * ((g.ny + 3) / 4);
-    int nch = 1;
-    while (cols * nch < 4096 && g.nzg / (nch * 2) >= 16) nch *= 2;
-    return (g.nzg + nch - 1) / nch;
-}
 
 extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt, void *s,
                                 void *qt, void *wt, void *T, void *qv, void *ql, void *alpha, void *buoyancy,
@@ -193,8 +186,9 @@ extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, 
     a.mlp = (const mlp::Weights *)opt->mlp;
     a.newton = opt->newton_steps;
     a.counters = (unsigned long long *)opt->counters;
-    a.kchunk = thermo_kchunk(g);
-    const int nch = (g.nzg + a.kchunk - 1) / a.kchunk;
+    const ZSplit zs = thermo_zsplit(g.nx, g.ny, g.nzg);
+    a.kchunk = zs.len;
+    const int nch = zs.nch;
     dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, nch);
     hipStream_t st = (hipStream_t)stream;
     if (precision == PYCLES_F64) {
@@ -287,8 +281,9 @@ extern "C" int pycles_thermo_dry(const pycles_dims *dims, const pycles_ref *ref,
     a.s = s; a.wt = wt; a.T = T; a.alpha = alpha; a.buoy = buoyancy; a.theta = theta; a.bvf = bvf;
     a.sbase = (precision == PYCLES_F32 && base) ? base->base[0] : nullptr;
     a.wt_over = wt_overwrite;
-    a.kchunk = thermo_kchunk(g);
-    const int nch = (g.nzg + a.kchunk - 1) / a.kchunk;
+    const ZSplit zs = thermo_zsplit(g.nx, g.ny, g.nzg);
+    a.kchunk = zs.len;
+    const int nch = zs.nch;
     dim3 blk(64, 4), grd((g.nx + 63) / 64, (g.ny + 3) / 4, nch);
     hipStream_t st = (hipStream_t)stream;
     if (precision == PYCLES_F64)

@@ -103,7 +103,15 @@ __global__ __launch_bounds__(256, MLP ? 4 : 5) void thermo_sa_f32_kernel(Geo g, 
         double qt64 = qb + (double)qd;
         if (a.clip && !(qt64 >= 1e-11)) {
             qt64 = 1e-11;
-            if (own) qtp[o] = (float)(1e-11 - qb);
+            if (own) {
+                // the stored deviation is stepped down until qb + dev, formed
+                // as every reader forms it, is below 1e-11: a neighbouring
+                // chunk that reads this level after the store clips it again
+                // to exactly 1e-11, as one that reads it before does
+                float dev = (float)(1e-11 - qb);
+                for (int t = 0; t < 4 && !(qb + (double)dev < 1e-11); ++t) dev = nextafterf(dev, -INFINITY);
+                qtp[o] = dev;
+            }
         }
         const float qt = (float)qt64;
         const double s64 = sb + (double)sd;

@@ -337,17 +337,9 @@ __global__ __launch_bounds__(256) void cfl_kernel(Geo g, const T *u, const T *v,
         const long long o = t + (long long)g.gw * g.plane;
         const double c = dt * (fabs((double)u[o]) * g.dxi[0] + fabs((double)v[o]) * g.dxi[1] +
                                fabs((double)w[o]) * g.dxi[2]);
-        m = fmax(m, c);
+        m = cfl_fold(m, c);
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        atomicMax((unsigned long long *)out, (unsigned long long)__double_as_longlong(red[0]));
+    cfl_block_max(m, out);
 }
 
 extern "C" int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w, double dt,
@@ -419,6 +411,14 @@ extern "C" int pycles_layout_ingress(const pycles_dims *dims, const double *ref_
                                      const double *base, int precision, void *stream) {
     int rc = pycles_check_dims(dims);
     if (rc) return rc;
+    if (!ref_layout || !dev) {
+        pycles_set_error("layout_ingress: NULL array");
+        return PYCLES_ERR_ARG;
+    }
+    if (precision != PYCLES_F64 && precision != PYCLES_F32) {
+        pycles_set_error("layout_ingress: bad precision %d", precision);
+        return PYCLES_ERR_ARG;
+    }
     Geo g = make_geo(dims);
     hipStream_t st = (hipStream_t)stream;
     if (precision == PYCLES_F64)
@@ -433,6 +433,14 @@ extern "C" int pycles_layout_egress(const pycles_dims *dims, const void *dev, do
                                     const double *base, int precision, void *stream) {
     int rc = pycles_check_dims(dims);
     if (rc) return rc;
+    if (!ref_layout || !dev) {
+        pycles_set_error("layout_egress: NULL array");
+        return PYCLES_ERR_ARG;
+    }
+    if (precision != PYCLES_F64 && precision != PYCLES_F32) {
+        pycles_set_error("layout_egress: bad precision %d", precision);
+        return PYCLES_ERR_ARG;
+    }
     Geo g = make_geo(dims);
     hipStream_t st = (hipStream_t)stream;
     if (precision == PYCLES_F64)
@@ -440,5 +448,52 @@ extern "C" int pycles_layout_egress(const pycles_dims *dims, const void *dev, do
     else
         hipLaunchKernelGGL(egress_kernel<float>, dim3(2048), dim3(256), 0, st, g, (const float *)dev, ref_layout, base);
     PYCLES_CHECK_LAUNCH();
+    return PYCLES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The CFL value on the host, after the stream's work: the time-step
+// adjustment's one synchronisation per step (TimeStepping.pyx:265-266).
+extern "C" int pycles_cfl_read(const double *dev_cfl, double *host_cfl, void *stream) {
+    if (!dev_cfl || !host_cfl) {
+        pycles_set_error("cfl_read: NULL argument");
+        return PYCLES_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(host_cfl, dev_cfl, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        pycles_set_error("cfl_read: %s", hipGetErrorString(e));
+        return PYCLES_ERR_HIP;
+    }
+    const double h = *host_cfl;
+    if (!(h - h == 0.0)) {  // NaN or infinite
+        pycles_set_error("CFL number is not finite (%g): a velocity is NaN or infinite", *host_cfl);
+        return PYCLES_ERR_NONFINITE;
+    }
+    return PYCLES_OK;
+}
+
+// Chunk layout of the z-chunked kernels (see the header).
+extern "C" int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len) {
+    int rc = pycles_check_dims(dims);
+    if (rc) return rc;
+    if (!nchunks || !chunk_len) {
+        pycles_set_error("launch_chunks: NULL argument");
+        return PYCLES_ERR_ARG;
+    }
+    const Geo g = make_geo(dims);
+    ZSplit z;
+    switch (kernel) {
+    case 0: z = correct_zsplit(g.plane, g.nz, g.gw); break;
+    case 1: z = advy_zsplit(g.nx, g.ny, g.nz); break;
+    case 2: z = advz_zsplit(g.nx, g.ny, g.nz); break;
+    case 3: z = thermo_zsplit(g.nx, g.ny, g.nzg); break;
+    default:
+        pycles_set_error("launch_chunks: unknown kernel %d", kernel);
+        return PYCLES_ERR_ARG;
+    }
+    *nchunks = z.nch;
+    *chunk_len = z.len;
     return PYCLES_OK;
 }

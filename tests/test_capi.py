@@ -96,3 +96,46 @@ def test_pressure_plan_rejects_unsupported_sizes(lib):
     rc = lib.pycles_pressure_plan_create(C.byref(_dims(n=(16, 16, 2049))), rho, rho, 0, C.byref(plan))
     assert rc == -3 and not plan.value
     assert b"2048" in lib.pycles_last_error()
+
+
+def _chunks(lib, kernel, n, gw=3):
+    nch, ln = C.c_int(), C.c_int()
+    lib.pycles_launch_chunks.restype = C.c_int
+    rc = lib.pycles_launch_chunks(kernel, C.byref(_dims(n=n, gw=gw)), C.byref(nch), C.byref(ln))
+    assert rc == 0
+    return nch.value, ln.value
+
+
+def test_z_chunks_partition_every_level_count(lib):
+    """The z chunks of the chunked kernels, as their grids use them (the
+    formulae in the header), cover each level exactly once with no empty
+    chunk; the pressure correction's chunks hold at least gw levels each
+    (its first and last chunks fill the ghost levels from them)."""
+    for gw in (3, 5):
+        for plane in ((64, 64), (512, 512), (16, 8)):
+            for nz in range(max(2, gw), 2049):
+                n = (plane[0], plane[1], nz)
+                nch, ln = _chunks(lib, 0, n, gw)
+                ranges = [(gw + b * ln, nz + gw if b + 1 == nch else gw + (b + 1) * ln) for b in range(nch)]
+                assert ranges[0][0] == gw and ranges[-1][1] == nz + gw
+                assert all(r[1] == q[0] for r, q in zip(ranges, ranges[1:]))
+                lens = [r[1] - r[0] for r in ranges]
+                assert min(lens) >= (gw if nch > 1 else 1), (gw, n, ranges)
+                for kern, total in ((1, plane[1]), (2, nz), (3, nz + 2 * gw)):
+                    nch, ln = _chunks(lib, kern, n, gw)
+                    ranges = [(b * ln, min((b + 1) * ln, total)) for b in range(nch)]
+                    assert ranges[-1][1] == total and all(r[1] > r[0] for r in ranges), (kern, n, ranges)
+
+
+def test_layout_and_cfl_read_reject_bad_arguments(lib):
+    lib.pycles_layout_ingress.restype = C.c_int
+    lib.pycles_layout_egress.restype = C.c_int
+    lib.pycles_cfl_read.restype = C.c_int
+    fake = C.c_void_p(0x1000)      # never dereferenced: the checks fail first
+    d = _dims()
+    assert lib.pycles_layout_ingress(C.byref(d), None, fake, None, 0, None) == -1
+    assert lib.pycles_layout_ingress(C.byref(d), fake, fake, None, 3, None) == -1
+    assert lib.pycles_layout_egress(C.byref(d), fake, None, None, 0, None) == -1
+    assert lib.pycles_layout_egress(C.byref(d), fake, fake, None, -1, None) == -1
+    assert lib.pycles_cfl_read(None, None, None) == -1
+    assert lib.pycles_launch_chunks(9, C.byref(d), C.byref(C.c_int()), C.byref(C.c_int())) == -1

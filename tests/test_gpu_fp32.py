@@ -76,3 +76,39 @@ def test_fp32_thermo_clip_and_saturation(gpu):
     others = np.delete(np.arange(gw, qt.shape[0] - gw), 2)
     qt_o = fields["qt"]
     assert np.max(np.abs(qt[others] - qt_o[others]) / np.abs(qt_o[others])) < 1e-6
+
+
+def test_fp32_qt_clip_is_idempotent_across_chunks(gpu):
+    """Clipped levels at the edge between two z chunks of the FP32
+    thermodynamics kernel: a chunk reads its neighbour's edge level, which the
+    owner may already have clipped and stored.  The stored deviation re-clips
+    to exactly 1e-11 in every reader, so a second update on the clipped state
+    gives the first update's outputs bit for bit (before, the stored FP32
+    deviation could read back above 1e-11 and change T, b and N^2 there)."""
+    import json
+    from pycles_amd import _capi, _device as D
+    from pycles_amd.Simulation3d import Simulation3d
+    from tests.test_gpu_parity import _saturating
+    nl, Gr, RS, Th, fields = host_case(32, 24, 40, seed=4)
+    fields = _saturating(Gr, fields)
+    nch, ln = _capi.launch_chunks(3, D.dims_of(Gr))
+    assert nch > 1
+    q = fields["qt"].copy()
+    for k in (ln - 1, ln, ln + 1):
+        q[k] = -1e-4
+    fields["qt"] = q
+    nl = json.loads(json.dumps(nl))
+    nl["mi355x"]["precision"] = "f32"
+    sim = Simulation3d().initialize(nl, Gr=Gr, Ref=RS, Th=Th, fields=fields)
+    sim.Th.max_iter = 0
+    out = []
+    for _ in range(2):
+        sim.Th.update(sim.Gr, sim.Ref, sim.PV, sim.DV, wt_overwrite=True)
+        torch.cuda.synchronize()
+        out.append((sim.DV.values.clone(), sim.PV.tend("w").clone(), sim.PV.field("qt").clone()))
+    assert torch.equal(out[0][0], out[1][0])
+    assert torch.equal(out[0][1], out[1][1])
+    assert torch.equal(out[0][2], out[1][2])
+    qt = sim.PV.get_device_layout("qt")
+    for k in (ln - 1, ln, ln + 1):
+        assert np.all(qt[k] < 1e-11) and np.all(np.abs(qt[k] - 1e-11) < 2e-9)

@@ -176,12 +176,14 @@ def test_rk3_stage_and_bcs_bitwise(gpu, rk):
     assert float(sim.PV.tendencies.abs().max()) == 0.0
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(32, 16, 24), (8, 64, 24), (16, 128, 24), (64, 8, 24), (256, 32, 24),
-                                      (32, 16, 13), (16, 8, 5)])
-def test_pressure_parity(gpu, nx, ny, nz):
+@pytest.mark.parametrize("nx,ny,nz,gw", [(32, 16, 24, 3), (8, 64, 24, 3), (16, 128, 24, 3), (64, 8, 24, 3),
+                                         (256, 32, 24, 3), (32, 16, 13, 3), (16, 8, 5, 3), (64, 64, 50, 3),
+                                         (32, 16, 100, 3), (16, 16, 97, 5)])
+def test_pressure_parity(gpu, nx, ny, nz, gw):
     # nx, ny cover first Stockham passes of radix 2, 4 and 8 in both directions;
-    # nz = 13 and 5 leave tails after the Thomas sweeps' blocks of 8 levels
-    nl, Gr, RS, Th, fields = host_case(nx, ny, nz, seed=9)
+    # nz = 13 and 5 leave tails after the Thomas sweeps' blocks of 8 levels;
+    # nz = 50, 100, 97 split the correction's z chunks with a remainder
+    nl, Gr, RS, Th, fields = host_case(nx, ny, nz, gw=gw, seed=9)
     fields = _swirl(Gr, fields, amp=2.0)
     sim = _gpu_sim(nl, Gr, RS, Th, fields)
     sim.Pr.update(sim.Gr, sim.Ref, sim.DV, sim.PV, sim.Pa)

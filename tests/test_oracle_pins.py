@@ -456,3 +456,264 @@ def test_sa_eos_saturated_round_trip(cc_table):
         # update moves by < 1e-3 K (d ln pv*/dT < 0.08 / K)
         assert pv2 == pytest.approx(lt(T2), rel=1e-4)
         assert abs(L.orc_entropy_sa(p0, T2, qt, ql2, 0.0) - s) < 1e-3 * 10.0
+
+
+# --------------------------------------------------------------------------
+# Thermodynamic diagnostics, latent heat and the scalar flux's density weights
+
+def _thermo_lib():
+    L = O.lib()
+    D = C.POINTER(O.OrcDims)
+    L.orc_buoyancy_update_sa.argtypes = [D, O._dp, O._dp, O._dp, O._dp]
+    L.orc_buoyancy_update_dry.argtypes = [D, O._dp, O._dp, O._dp, O._dp]
+    L.orc_bvf_sa.argtypes = [D, C.POINTER(O.OrcLookup), O._dp, O._dp, O._dp, O._dp, O._dp, O._dp]
+    L.orc_bvf_dry.argtypes = [D, O._dp, O._dp, O._dp, O._dp]
+    L.orc_thetali_update.argtypes = [D, O._dp, O._dp, O._dp, O._dp, O._dp, O._dp]
+    return L
+
+
+def _col(d, prof):
+    """A field equal to a z profile in every column (reference layout)."""
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(prof, dtype=np.float64), tuple(d.nlg)))
+
+
+@pytest.mark.parametrize("moist", [True, False])
+def test_buoyancy_is_zero_at_reference_and_face_averaged(moist):
+    """b = g (alpha - alpha0_half)/alpha0_half: zero where alpha is the
+    reference alpha0_half; for b linear in k the w tendency gains the face
+    average beta (k + 1/2), exactly on the k range of the reference loops
+    (SA: i, j all, k in [1, nlg-3), thermodynamics_sa.h:118-128; dry: i, j in
+    [0, nlg-1), k in [0, nlg-1), thermodynamics_dry.h:62-70) and nowhere else."""
+    L = _thermo_lib()
+    f = L.orc_buoyancy_update_sa if moist else L.orc_buoyancy_update_dry
+    d = O.make_dims(4, 3, 8, 3, 50.0, 50.0, 20.0)
+    n0, n1, n2 = d.nlg
+    rng = np.random.default_rng(31)
+    a0h = rng.uniform(0.8, 1.2, n2)
+    wt0 = rng.normal(size=tuple(d.nlg))
+    # at the reference state (the buoyancy array starts at zero, as the
+    # diagnostic variables are allocated)
+    b = np.zeros(tuple(d.nlg))
+    wt = wt0.copy()
+    f(C.byref(d), O.ptr(a0h), O.ptr(_col(d, a0h)), O.ptr(b), O.ptr(wt))
+    assert np.array_equal(wt, wt0)
+    # b = beta k
+    beta = 0.37
+    k = np.arange(n2, dtype=np.float64)
+    alpha = _col(d, a0h * (1.0 + beta * k / P.g))
+    b = np.zeros(tuple(d.nlg))
+    wt = wt0.copy()
+    f(C.byref(d), O.ptr(a0h), O.ptr(alpha), O.ptr(b), O.ptr(wt))
+    ib, jb = (slice(None), slice(None)) if moist else (slice(0, n0 - 1), slice(0, n1 - 1))
+    assert np.allclose(b[ib, jb, :n2 - 1], beta * k[:n2 - 1], rtol=0, atol=1e-12)
+    # no loop writes the top level, and the dry average at k = nlg-2 reads it
+    # as the allocation left it (thermodynamics_dry.h:51 and :67)
+    assert np.all(b[:, :, n2 - 1] == 0.0)
+    kr = np.arange(1, n2 - 3) if moist else np.arange(0, n2 - 1)
+    bk = np.where(k < n2 - 1, beta * k, 0.0)
+    want = wt0.copy()
+    want[ib, jb, kr] += 0.5 * (bk[kr] + bk[kr + 1])
+    assert np.allclose(wt, want, rtol=0, atol=1e-12)
+    untouched = np.ones(tuple(d.nlg), dtype=bool)
+    untouched[ib, jb, kr] = False
+    assert np.array_equal(wt[untouched], wt0[untouched])
+
+
+def test_buoyancy_uses_alpha0_half_in_the_update(cc_table):
+    """Through ThermodynamicsSA.update: an unsaturated state whose alpha is
+    the half-level reference alpha0_half has zero buoyancy although the
+    full-level alpha0 differs by 10%."""
+    _, lt = cc_table
+    gw = 3
+    d = O.make_dims(3, 3, 6, gw, 50.0, 50.0, 20.0)
+    n2 = d.nlg[2]
+    p0h = np.linspace(1.0e5, 9.6e4, n2)
+    T0 = np.linspace(298.0, 295.0, n2)
+    qt = 0.008
+    a0h = np.array([O.lib().orc_alpha(p, t, qt, qt) for p, t in zip(p0h, T0)])
+    ref = O.Ref(p0h, p0h, 1.1 * a0h, a0h, 1.0 / (1.1 * a0h), 1.0 / a0h)
+    sim = O.OracleSimulation(d, ref, True, lt)
+    sim.pv("s")[...] = _col(d, [O.lib().orc_entropy_sa(p, t, qt, 0.0, 0.0) for p, t in zip(p0h, T0)])
+    sim.pv("qt")[...] = qt
+    sim.thermo_update()
+    assert np.abs(sim.dv["buoyancy"][:, :, :n2 - 1]).max() < 1e-11
+    assert np.abs(sim.pt("w")).max() < 1e-11
+
+
+def test_bvf_unsaturated_is_g_over_theta_rho_times_its_gradient(cc_table):
+    """Unsaturated: N^2 = g/theta_rho dtheta_rho/dz, with the gradient the
+    centred difference of face averages; theta_rho linear in z gives exactly
+    g gamma / theta_rho on k in [2, nlg-2) (thermodynamics_sa.h:163-178),
+    theta_rho on [1, nlg-1), nothing elsewhere.  theta_rho is the density
+    potential temperature theta (1 + r_v/eps_v)/(1 + r_t) (mixing ratios)."""
+    _, lt = cc_table
+    L = _thermo_lib()
+    d = O.make_dims(3, 2, 10, 3, 50.0, 50.0, 20.0)
+    n2 = d.nlg[2]
+    p0 = np.linspace(1.0e5, 9.0e4, n2)
+    gamma = 3.5e-3
+    theta_rho = 300.0 + gamma * 20.0 * np.arange(n2)
+    qt = np.linspace(0.012, 0.006, n2)
+    qv = qt.copy()
+    ex = np.array([O.lib().orc_exner(p) for p in p0])
+    rv, rt = qv / (1 - qt), qt / (1 - qt)
+    T = theta_rho * ex * (1 + rt) / (1 + rv / P.eps_v)
+    thr = np.full(tuple(d.nlg), -7.0)
+    bvf = np.full(tuple(d.nlg), -7.0)
+    L.orc_bvf_sa(C.byref(d), C.byref(lt.c), O.ptr(p0), O.ptr(_col(d, T)), O.ptr(_col(d, qt)),
+                 O.ptr(_col(d, qv)), O.ptr(thr), O.ptr(bvf))
+    assert np.allclose(thr[:, :, 1:n2 - 1], theta_rho[1:n2 - 1], rtol=1e-14, atol=0)
+    assert np.all(thr[:, :, 0] == -7.0) and np.all(thr[:, :, n2 - 1] == -7.0)
+    k = np.arange(2, n2 - 2)
+    assert np.allclose(bvf[:, :, k], P.g * gamma / theta_rho[k], rtol=1e-9, atol=0)
+    for kk in (0, 1, n2 - 2, n2 - 1):
+        assert np.all(bvf[:, :, kk] == -7.0)
+
+
+def test_bvf_saturated_against_durran_klemp(cc_table):
+    """Saturated (qv < qt): N^2 is affine in dT/dz and in dqt/dz.  Its slope
+    in dT/dz is g/T (1 + L r_s/(Rd T)), and its value at dT/dz = dqt/dz = 0
+    over that slope is the moist-adiabatic lapse rate, which must agree with
+    Durran & Klemp's (1982, eq. 19) Gamma_m = g (1 + r_w)(1 + L r_s/(Rd T)) /
+    (cpd + cpv r_s + cl r_l + L^2 r_s (eps + r_s)/(Rd T^2)) to the few percent
+    by which the reference's specific-humidity form differs; N^2 then vanishes
+    at dT/dz = -Gamma.  The dqt/dz slope is the reference's -1/(1 - qt)
+    (thermodynamics_sa.h:170, without the factor g of Durran & Klemp's term)."""
+    cc, lt = cc_table
+    L = _thermo_lib()
+    d = O.make_dims(2, 2, 6, 3, 50.0, 50.0, 20.0)
+    n2 = d.nlg[2]
+    kc = n2 // 2
+    dz = 20.0
+    for p0c, Tc, qtc in ((9.0e4, 290.0, 0.02), (8.0e4, 280.0, 0.012), (9.8e4, 300.0, 0.03)):
+        p0 = np.full(n2, p0c)
+        pvs = lt(Tc)
+        qvs = P.eps_v * (1 - qtc) * pvs / (p0c - pvs)
+        assert qvs < qtc
+
+        def n2_at(dTdz, dqdz):
+            T = np.full(n2, Tc)
+            q = np.full(n2, qtc)
+            T[kc + 1] += dTdz * dz
+            T[kc - 1] -= dTdz * dz
+            q[kc + 1] += dqdz * dz
+            q[kc - 1] -= dqdz * dz
+            thr = np.zeros(tuple(d.nlg))
+            bvf = np.zeros(tuple(d.nlg))
+            L.orc_bvf_sa(C.byref(d), C.byref(lt.c), O.ptr(p0), O.ptr(_col(d, T)), O.ptr(_col(d, q)),
+                         O.ptr(_col(d, np.full(n2, qvs))), O.ptr(thr), O.ptr(bvf))
+            return bvf[0, 0, kc]
+
+        n0 = n2_at(0.0, 0.0)
+        slope = (n2_at(1e-3, 0.0) - n0) / 1e-3
+        Lv = O.lib().orc_latent_heat_variable(Tc)
+        rs = P.eps_v * pvs / (p0c - pvs)
+        assert slope == pytest.approx(P.g / Tc * (1 + Lv * rs / (P.Rd * Tc)), rel=1e-8)
+        gam = n0 / slope
+        rw = qtc / (1 - qtc)
+        cl = 4218.0
+        dk = P.g * (1 + rw) * (1 + Lv * rs / (P.Rd * Tc)) / (
+            P.cpd + P.cpv * rs + cl * (rw - rs) + Lv * Lv * rs * (P.eps_v + rs) / (P.Rd * Tc * Tc))
+        assert gam == pytest.approx(dk, rel=0.05), (gam, dk)
+        assert abs(n2_at(-gam, 0.0)) < 1e-9 * abs(n0)
+        qslope = (n2_at(0.0, 1e-6) - n0) / 1e-6
+        assert qslope == pytest.approx(-1.0 / (1 - qtc), rel=1e-6)
+
+
+def test_thetali_is_theta_without_condensate_and_follows_betts(cc_table):
+    """theta_l = T/Pi exactly when ql = qi = 0; its sensitivity to liquid
+    water at ql -> 0 is Betts's (1973) -L/(cpd T), up to the 1/(1 - qt) by
+    which the reference's specific-humidity form differs."""
+    L = _thermo_lib()
+    d = O.make_dims(2, 2, 4, 3, 50.0, 50.0, 20.0)
+    n2 = d.nlg[2]
+    p0 = np.linspace(1.0e5, 9.0e4, n2)
+    T = np.linspace(295.0, 289.0, n2)
+    qt = 0.015
+    out = np.zeros(tuple(d.nlg))
+    z = np.zeros(tuple(d.nlg))
+    L.orc_thetali_update(C.byref(d), O.ptr(p0), O.ptr(_col(d, T)), O.ptr(_col(d, np.full(n2, qt))), O.ptr(z),
+                         O.ptr(z), O.ptr(out))
+    theta = T / np.array([O.lib().orc_exner(p) for p in p0])
+    assert np.array_equal(out[0, 0], theta)
+    h = 1e-7
+    out2 = np.zeros(tuple(d.nlg))
+    L.orc_thetali_update(C.byref(d), O.ptr(p0), O.ptr(_col(d, T)), O.ptr(_col(d, np.full(n2, qt))),
+                         O.ptr(_col(d, np.full(n2, h))), O.ptr(z), O.ptr(out2))
+    dln = (np.log(out2[0, 0]) - np.log(out[0, 0])) / h
+    Lv = np.array([O.lib().orc_latent_heat_variable(t) for t in T])
+    assert np.allclose(dln, -Lv / (P.cpd * T) / (1 - qt), rtol=1e-5)
+    assert np.allclose(dln, -Lv / (P.cpd * T), rtol=0.02)
+
+
+def test_latent_heat_fit():
+    """L(T) (Microphysics.pxd:71-75): 2.5008e6 J/kg at 0 C; the tabulated
+    values of Rogers & Yau (1989, table 2.1) at -20, 20 and 30 C to 0.05%;
+    dL/dT at 0 C equal to cpv - cl (Kirchhoff, cl = 4218 J/kg/K) to 1%.
+    Where the reference tree is present, its polynomial's coefficients are read
+    from the source text and evaluated against the oracle."""
+    import os
+    import re
+    L = O.lib().orc_latent_heat_variable
+    assert L(273.15) == 2.5008e6
+    for tc, want in ((-20.0, 2.549e6), (20.0, 2.453e6), (30.0, 2.430e6)):
+        assert L(273.15 + tc) == pytest.approx(want, rel=5e-4)
+    dLdT = (L(273.15 + 0.01) - L(273.15 - 0.01)) / 0.02
+    assert dLdT == pytest.approx(P.cpv - 4218.0, rel=0.01)
+    src = "/root/reference/Microphysics.pxd"
+    if os.path.exists(src):
+        text = open(src).read()
+        m = re.search(r"latent_heat_variable\(double T.*?return\s*\((.*?)\)\s*\*\s*1000\.0", text, re.S)
+        assert m, "latent_heat_variable not found in the reference text"
+        expr = " ".join(m.group(1).split())
+        nums = [float(x) for x in re.findall(r"\d+\.\d+", expr)]
+        assert nums == [2500.8, 2.36, 0.0016, 0.00006], expr
+        for T in (250.0, 273.15, 290.0, 305.0):
+            tc = T - 273.15
+            assert L(T) == (2500.8 - 2.36 * tc + 0.0016 * tc * tc - 0.00006 * tc * tc * tc) * 1000.0
+
+
+@pytest.mark.parametrize("direction", [0, 2])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_scalar_flux_density_weights(direction, sign):
+    """Quadratic phi under uniform velocity U.  The finite-difference WENO
+    face value of point values is exact for the function whose sliding cell
+    average is phi, h(x) = phi(x) - c2 dx^2/12 for phi = c0 + c1 x + c2 x^2,
+    so the tendency is the flux-form difference with the reference's weights,
+    rho0 (full levels, the z faces) in z and rho0_half in x and y
+    (scalar_advection.h:361 and :388), times alpha0_half
+    (flux_divergence.h:4-32): -a0h[k] U (rho0[k] h(k+1/2) - rho0[k-1] h(k-1/2))/dz
+    in z, -a0h[k] rho0_half[k] U (h(i+1/2) - h(i-1/2))/dx in x.  With rho
+    varying in z the constant -c2 dx^2/12 does not cancel, so the face
+    weights are seen."""
+    order = 5
+    r = 3
+    gw = r
+    n = (9, 8, 10)
+    dx = (30.0, 45.0, 20.0)
+    nlg2 = n[2] + 2 * gw
+    rng = np.random.default_rng(91 + direction)
+    rho0 = rng.uniform(0.7, 1.2, nlg2)
+    rho0h = rng.uniform(0.7, 1.2, nlg2)
+    sim, d = _dry_sim(*n, gw, dx, rho0, rho0h, order)
+    coef = rng.normal(size=r) / np.array([1.0, 5.0, 25.0])
+    idx = np.indices(tuple(d.nlg), dtype=np.float64)[direction]
+    xs = (idx - gw) * dx[direction] / 100.0
+    poly = lambda x: np.polynomial.polynomial.polyval(x, coef)  # noqa: E731
+    sim.pv("s")[...] = poly(xs)
+    U = 2.0 * sign
+    sim.pv(("u", "v", "w")[direction])[...] = U
+    sim.scalar_advection_update()
+    h = dx[direction] / 100.0
+    face = lambda x: poly(x) - coef[2] * h * h / 12.0  # noqa: E731
+    up, dn = face(xs + 0.5 * h), face(xs - 0.5 * h)
+    a0h = (1.0 / rho0h)[None, None, :]
+    if direction == 2:
+        rk = rho0[None, None, :]
+        rkm = np.roll(rho0, 1)[None, None, :]
+        want = -a0h * U * (rk * up - rkm * dn) / dx[2]
+    else:
+        want = -a0h * rho0h[None, None, :] * U * (up - dn) / dx[0]
+    sl = (slice(gw, -gw),) * 3
+    got = sim.pt("s")[sl]
+    assert np.max(np.abs(got - want[sl])) <= 1e-11 * np.max(np.abs(want[sl]))
