@@ -225,6 +225,49 @@ int pycles_layout_ingress(const pycles_dims *dims, const double *ref_layout, voi
 int pycles_layout_egress(const pycles_dims *dims, const void *dev, double *ref_layout,
                          const double *base, int precision, void *stream);
 
+/* ---- statistics --------------------------------------------------------- */
+/* Horizontal statistics per level (ParallelMPI.HorizontalMean,
+ * HorizontalMeanofSquares, HorizontalMeanofCubes, HorizontalMaximum,
+ * HorizontalMinimum, HorizontalMeanConditional and
+ * HorizontalMeanofSquaresConditional, ParallelMPI.pyx:196-540) in one pass.
+ * For every level k in [0, nzg), over the interior columns, with
+ * v = (a + base_a[k]) * (b + base_b[k]) (the second factor only when b is
+ * given; a NULL base counts as 0) and m = mask (1 where mask is NULL):
+ *   out[0*nzg + k] = sum v m / D,  out[1*nzg + k] = sum v^2 m / D,
+ *   out[2*nzg + k] = sum v^3 m / D,
+ *   out[3*nzg + k] = max(-9e12, max of v where m != 0),
+ *   out[4*nzg + k] = min(9e12, min of v where m != 0),  out[5*nzg + k] = sum m,
+ * with D = nx*ny (multiplied by its reciprocal, as the reference) without a
+ * mask and max(sum m, 1) with one.  a, b and mask are device-layout fields of
+ * the given precision; out is an FP64 device array of 6*nzg entries.  The
+ * sums are FP64 in a fixed order (deterministic, not the reference's serial
+ * order).  workspace: pycles_horizontal_stats_workspace_bytes(dims) bytes of
+ * device memory, overwritten. */
+size_t pycles_horizontal_stats_workspace_bytes(const pycles_dims *dims);
+int pycles_horizontal_stats(const pycles_dims *dims, const void *a, const double *base_a, const void *b,
+                            const double *base_b, const void *mask, int precision, double *out,
+                            void *workspace, void *stream);
+/* The fields of ThermodynamicsSA.stats_io (ThermodynamicsSA.pyx:256-389) at
+ * every stored point: thetas = thetas_c(s, qt), theta = T / exner(p0_half),
+ * rh = pv_c(p0_half, qt, qv) / pv*(T); outputs in the field precision, any
+ * of them may be NULL.  base: the FP32 storage profiles of s and qt. */
+int pycles_sa_stat_fields(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt,
+                          const void *s, const void *qt, const void *T, const void *qv,
+                          const pycles_scalar_base *base, int precision, void *thetas, void *theta,
+                          void *rh, void *stream);
+/* ThermodynamicsSA.liquid_stats (ThermodynamicsSA.pyx:395-492) of the
+ * liquid water ql: out[k] (k < nz) = fraction of cells with ql > 0 at
+ * interior level k; out[nz] = fraction of columns with any; out[nz+1],
+ * out[nz+2] = cloud base and top, the lowest and highest (k + 1/2) dz with
+ * a cloudy cell (99999.9 / -99999.9 if none); out[nz+3] = liquid water path,
+ * the column mean of sum_k rho0_half[k] ql dz with the interior level k
+ * weighted by the ghosted profile's k-th entry, as the reference indexes it.
+ * out: FP64 device array of nz + 4 entries.  workspace:
+ * pycles_cloud_stats_workspace_bytes(dims) bytes, overwritten.  nz <= 12287. */
+size_t pycles_cloud_stats_workspace_bytes(const pycles_dims *dims);
+int pycles_cloud_stats(const pycles_dims *dims, const double *rho0_half, const void *ql, int precision,
+                       double *out, void *workspace, void *stream);
+
 /* ---- launch geometry (diagnostic) -------------------------------------- */
 /* z chunks of the z-chunked kernels for dims: kernel 0 = pressure
  * correction (chunks of the interior levels [gw, nz+gw), the last one taking

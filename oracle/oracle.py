@@ -293,3 +293,84 @@ class OracleSimulation:
         lib().orc_divergence(C.byref(self.d), C.byref(self.ref.c), ptr(self.pv("u")), ptr(self.pv("v")),
                              ptr(self.pv("w")), ptr(div))
         return div
+
+
+# --------------------------------------------------------------------------
+# Statistics (numpy restatement; reference layout [i][j][k], ghosts on every
+# side, reductions over the interior columns and every level)
+
+def horizontal_stats(d: OrcDims, a, b=None, mask=None):
+    """ParallelMPI.HorizontalMean / MeanofSquares / MeanofCubes /
+    Maximum / Minimum / MeanConditional (ParallelMPI.pyx:196-540) of v = a
+    (times b): rows mean, mean of squares, mean of cubes, max, min, mask sum.
+    Sums in the reference's serial order (i, then j) per level."""
+    gw = d.gw
+    sl = (slice(gw, d.nlg[0] - gw), slice(gw, d.nlg[1] - gw))
+    v = np.asarray(a, dtype=np.float64)[sl]
+    if b is not None:
+        v = v * np.asarray(b, dtype=np.float64)[sl]
+    m = np.ones_like(v) if mask is None else np.asarray(mask, dtype=np.float64)[sl]
+    n2 = d.nlg[2]
+    out = np.zeros((6, n2))
+    flat_v = v.reshape(-1, n2)
+    flat_m = m.reshape(-1, n2)
+    for k in range(n2):
+        s1 = s2 = s3 = sm = 0.0
+        for x, w in zip(flat_v[:, k], flat_m[:, k]):
+            s1 += x * w
+            s2 += x * x * w
+            s3 += x * x * x * w
+            sm += w
+        out[0, k], out[1, k], out[2, k], out[5, k] = s1, s2, s3, sm
+        sel = flat_v[:, k][flat_m[:, k] != 0.0]
+        out[3, k] = max(-9e12, sel.max()) if sel.size else -9e12
+        out[4, k] = min(9e12, sel.min()) if sel.size else 9e12
+    if mask is None:
+        out[:3] *= 1.0 / float(d.n[0] * d.n[1])
+    else:
+        out[:3] /= np.maximum(out[5], 1.0)
+    return out
+
+
+def sa_stat_fields(p0_half, lt: Lookup, s, qt, T, qv):
+    """The fields of ThermodynamicsSA.stats_io (ThermodynamicsSA.pyx:256-389):
+    thetas_c(s, qt), theta_c(p0_half, T), pv_c(p0_half, qt, qv)/pv*(T)
+    (thermodynamic_functions.h:9-12, 23-25, 36-41)."""
+    from pycles_amd import parameters as P  # constants only
+    p0 = np.asarray(p0_half, dtype=np.float64)[None, None, :]
+    cpm = (1.0 - qt) * P.cpd + qt * P.cpv
+    thetas = P.T_tilde * np.exp((s - (1.0 - qt) * P.sd_tilde - qt * P.sv_tilde) / cpm)
+    theta = T / np.power(p0 / P.p_tilde, P.kappa)
+    pv_star = np.vectorize(lt.__call__)(T)
+    pv = p0 * P.eps_vi * qv / (1.0 - qt + P.eps_vi * qv)
+    return thetas, theta, pv / pv_star
+
+
+def liquid_stats(d: OrcDims, rho0_half, ql):
+    """ThermodynamicsSA.liquid_stats (ThermodynamicsSA.pyx:395-492) over the
+    interior: cloud fraction profile, cloud cover, cloud base and top (z_half
+    = (k + 1/2) dz), liquid water path with interior level k weighted by
+    rho0_half[k] as the reference indexes it (:478-480)."""
+    gw = d.gw
+    q = np.asarray(ql, dtype=np.float64)[gw:d.nlg[0] - gw, gw:d.nlg[1] - gw, gw:d.nlg[2] - gw]
+    nz = d.n[2]
+    div = float(d.n[0] * d.n[1])
+    cf = np.zeros(nz)
+    for k in range(nz):
+        for _ in range(np.count_nonzero(q[:, :, k] > 0.0)):
+            cf[k] += 1.0 / div          # one addition per cloudy cell (:447-451)
+    cover = float(np.count_nonzero((q > 0.0).any(axis=2))) / div
+    dz = d.dx[2]
+    cb, ct = 99999.9, -99999.9
+    for k in range(nz):
+        if np.any(q[:, :, k] > 0.0):
+            z = (k + 0.5) * dz
+            cb, ct = min(cb, z), max(ct, z)
+    lwp = 0.0
+    cols = q.reshape(-1, nz)
+    for col in cols:
+        c = 0.0
+        for k in range(nz):
+            c += rho0_half[k] * col[k] * dz
+        lwp += c
+    return cf, cover, cb, ct, lwp / div

@@ -50,6 +50,13 @@ class Grid:
         self.zl = self.z
         self.zl_half = self.z_half
 
+    def restart(self, Re):
+        """Grid.restart (Grid.pyx:184-203)."""
+        d = self.dims
+        Re.restart_data["Gr"] = {"dims": d.dims, "n": np.array(d.n), "ng": np.array(d.ng), "nl": np.array(d.nl),
+                                 "nlg": np.array(d.nlg), "indx_lo_g": np.array(d.indx_lo_g),
+                                 "indx_lo": np.array(d.indx_lo), "gw": d.gw, "dx": np.array(d.dx)}
+
     # ---- layout conversion (host numpy) ---------------------------------
     def to_device_layout(self, ref_field: np.ndarray) -> np.ndarray:
         """Reference ghosted [i][j][k] -> device [k][j][i] (x/y interior)."""

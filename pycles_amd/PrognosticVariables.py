@@ -160,6 +160,63 @@ class PrognosticVariables(_State):
     def zero_tendencies(self):
         self.tendencies.zero_()
 
+    # ---- statistics ------------------------------------------------------
+    def stats_io(self, Gr, RS, NS, Pa):
+        """PrognosticVariables.stats_io (PrognosticVariables.pyx:106-148):
+        mean, mean of squares and of cubes, max and min profiles of every
+        variable (one device pass each), the translational mean velocities and
+        the qt*s product mean."""
+        gw = Gr.dims.gw
+        for name in self.index_name:
+            st = Pa.horizontal_stats(Gr, self.field(name), self.precision, self.base.get(name))
+            NS.write_profile(name + "_mean", st[0, gw:-gw], Pa)
+            if name == "u":
+                NS.write_profile("u_translational_mean", st[0, gw:-gw] + RS.u0, Pa)
+            elif name == "v":
+                NS.write_profile("v_translational_mean", st[0, gw:-gw] + RS.v0, Pa)
+            NS.write_profile(name + "_mean2", st[1, gw:-gw], Pa)
+            NS.write_profile(name + "_mean3", st[2, gw:-gw], Pa)
+            NS.write_profile(name + "_max", st[3, gw:-gw], Pa)
+            NS.write_ts(name + "_max", np.amax(st[3, gw:-gw]), Pa)
+            NS.write_profile(name + "_min", st[4, gw:-gw], Pa)
+            NS.write_ts(name + "_min", np.amin(st[4, gw:-gw]), Pa)
+        if "qt" in self.name_index and "s" in self.name_index:
+            st = Pa.horizontal_stats(Gr, self.field("qt"), self.precision, self.base.get("qt"), self.field("s"),
+                                     self.base.get("s"))
+            NS.write_profile("qt_s_product_mean", st[0, gw:-gw], Pa)
+
+    # ---- restart ---------------------------------------------------------
+    def restart(self, Gr, Re):
+        """PrognosticVariables.restart (PrognosticVariables.pyx:269-314): the
+        registry and the values, here in the device layout [v][k][j][i] and
+        the storage precision (FP32: the deviations and their FP64 level
+        profiles), so that a resumed run continues bit for bit."""
+        Re.restart_data["PV"] = {
+            "name_index": dict(self.name_index), "units": dict(self.units),
+            "index_name": list(self.index_name), "nv": self.nv, "nv_scalars": self.nv_scalars,
+            "nv_velocities": self.nv_velocities, "bc_type": np.array(self.bc_type),
+            "var_type": np.array(self.var_type), "velocity_directions": np.array(self.velocity_directions),
+            "velocity_names_directional": list(self.velocity_names_directional),
+            "precision": "f32" if self.precision == D.F32 else "f64", "layout": "[v][k][j][i]",
+            "values": self.values.cpu().numpy(),
+            "base": {n: b.cpu().numpy() for n, b in self.base.items()},
+        }
+
+    def init_from_restart(self, Gr, Re):
+        """PrognosticVariables.init_from_restart (:316-357) into the already
+        registered and allocated state, which must match the file's."""
+        pv = Re.restart_data["PV"]
+        prec = "f32" if self.precision == D.F32 else "f64"
+        if list(pv["index_name"]) != list(self.index_name) or pv["precision"] != prec:
+            raise ValueError(f"restart file holds {pv['index_name']} in {pv['precision']}, "
+                             f"the run {self.index_name} in {prec}")
+        vals = np.asarray(pv["values"])
+        if tuple(vals.shape) != tuple(self.values.shape):
+            raise ValueError(f"restart values of shape {vals.shape}, the run's {tuple(self.values.shape)}")
+        self.values.copy_(torch.from_numpy(vals))
+        self.base = {n: D.to_device(np.asarray(b)) for n, b in pv["base"].items()}
+        self.tendencies.zero_()
+
     def val_nan(self, PA=None, message=""):
         if not torch.isfinite(self.values).all():
             raise FloatingPointError("Nans found in Prognostic Variables values " + message)
@@ -192,3 +249,16 @@ class DiagnosticVariables(_State):
 
     def get_variable_array(self, name, Gr) -> np.ndarray:
         return Gr.from_device_layout(self.get_device_layout(name))
+
+    def stats_io(self, Gr, NS, Pa):
+        """DiagnosticVariables.stats_io (DiagnosticVariables.pyx:155-185)."""
+        gw = Gr.dims.gw
+        for name in self.index_name:
+            st = Pa.horizontal_stats(Gr, self.field(name), self.precision)
+            NS.write_profile(name + "_mean", st[0, gw:-gw], Pa)
+            NS.write_profile(name + "_mean2", st[1, gw:-gw], Pa)
+            NS.write_profile(name + "_mean3", st[2, gw:-gw], Pa)
+            NS.write_profile(name + "_max", st[3, gw:-gw], Pa)
+            NS.write_ts(name + "_max", np.amax(st[3, gw:-gw]), Pa)
+            NS.write_profile(name + "_min", st[4, gw:-gw], Pa)
+            NS.write_ts(name + "_min", np.amin(st[4, gw:-gw]), Pa)

@@ -64,3 +64,20 @@ class ReferenceState:
         self.rho0 = 1.0 / alpha
         self.rho0_half = 1.0 / alpha_half
         return self
+
+    def restart(self, Gr, Re):
+        """ReferenceState.restart (ReferenceState.pyx:202-223)."""
+        Re.restart_data["Ref"] = {n: np.array(getattr(self, n)) for n in ("p0", "p0_half", "alpha0", "alpha0_half")}
+        Re.restart_data["Ref"].update({n: float(getattr(self, n, 0.0)) for n in ("Tg", "Pg", "sg", "qtg", "u0", "v0")})
+
+    def init_from_restart(self, Gr, Re):
+        """ReferenceState.init_from_restart (:225-245): rho0 = 1/alpha0."""
+        r = Re.restart_data["Ref"]
+        for n in ("Tg", "Pg", "sg", "qtg", "u0", "v0"):
+            setattr(self, n, r[n])
+        for n in ("p0", "p0_half", "alpha0", "alpha0_half"):
+            setattr(self, n, np.array(r[n], dtype=np.float64))
+        self.rho0 = 1.0 / self.alpha0
+        self.rho0_half = 1.0 / self.alpha0_half
+        self._device_ref = None
+        return self

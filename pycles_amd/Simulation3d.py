@@ -21,6 +21,7 @@ from .ParallelMPI import NetCDFIO_Stats, ParallelMPI
 from .PressureSolver import PressureSolver
 from .PrognosticVariables import DiagnosticVariables, PrognosticVariables
 from .ReferenceState import ReferenceState
+from .Restart import Restart
 from .Thermodynamics import LatentHeat
 from .ThermodynamicsSA import ThermodynamicsFactory
 from .TimeStepping import TimeStepping
@@ -52,7 +53,9 @@ class Simulation3d:
         self.Th = Th if Th is not None else ThermodynamicsFactory(namelist, None, self.LH, self.Pa)
         self.Ref = Ref if Ref is not None else ReferenceState(self.Gr)
         self.TS = TimeStepping()
-        self.StatsIO = NetCDFIO_Stats()
+        self.StatsIO = NetCDFIO_Stats().initialize(namelist, None, None)
+        self.Restart = Restart(namelist, self.Pa)
+        self.Restart.initialize()
         self.PV.add_variable("u", "m/s", "u", "u velocity component", "sym", "velocity", self.Pa)
         self.PV.set_velocity_direction("u", 0, self.Pa)
         self.PV.add_variable("v", "m/s", "v", "v velocity component", "sym", "velocity", self.Pa)
@@ -64,7 +67,19 @@ class Simulation3d:
         self.SA.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
         self.MA.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
         self.TS.initialize(namelist, self.PV, self.Pa)
-        if fields is None:
+        if self.Restart.is_restart_run:
+            # Simulation3d.pyx:103-121: state, time and reference from the file
+            self.Restart.read(self.Pa)
+            self.TS.t = self.Restart.restart_data["TS"]["t"]
+            self.TS.dt = self.Restart.restart_data["TS"]["dt"]
+            self.Ref.init_from_restart(self.Gr, self.Restart)
+            self.PV.init_from_restart(self.Gr, self.Restart)
+            if hasattr(self.Th, "init_from_restart"):
+                self.Th.init_from_restart(self.Restart)
+            self.Restart.last_restart_time = self.Restart.restart_data["last_restart_time"]
+            self.Restart.free_memory()
+            fields = {}
+        elif fields is None:
             fields = cases.initialize(namelist, self.Gr, self.PV, self.Ref, self.Th)
         else:
             for name, arr in fields.items():
@@ -88,7 +103,12 @@ class Simulation3d:
                           D.stream())
         self.Th.update(self.Gr, self.Ref, self.PV, self.DV)
         # the buoyancy just added stays in the w tendency for the first stage
-        self._wt_pending = True
+        # (the reference's run() does this on every start).  A restarted run
+        # continues as the run that wrote the file would have, whose w
+        # tendency started the stage at zero: it is cleared again.
+        if self.Restart.is_restart_run:
+            self.PV.zero_tendencies()
+        self._wt_pending = not self.Restart.is_restart_run
 
     def _mark(self, name):
         """Record a HIP event on the working stream when timing is enabled."""
@@ -148,6 +168,53 @@ class Simulation3d:
                        cfl_out=self.TS.cfl_buffer() if last else None, dt=self.TS.dt)
         self._mark("end")
         self.TS.adjust_timestep(self.Gr, self.PV, self.DV, self.Pa)
+        self.io()
+
+    def io(self):
+        """The restart part of Simulation3d.io (Simulation3d.pyx:191-283):
+        on the last stage, shorten dt to land on the next restart time, and
+        at that time dump the restart file (statistics output is the
+        Statistics module's, called by the caller)."""
+        TS, Re, NS = self.TS, self.Restart, self.StatsIO
+        stats_on = NS.path is not None
+        if not ((Re.enabled or stats_on) and TS.t > 0 and TS.rk_step == TS.n_rk_steps - 1):
+            return
+        dts = [TS.dt, TS.dt_max]
+        if stats_on:
+            dts += [NS.last_output_time + NS.frequency - TS.t, NS.frequency]
+        if Re.enabled:
+            dts += [Re.last_restart_time + Re.frequency - TS.t, Re.frequency]
+        TS.dt = min(x for x in dts if x > 0.0)
+        if stats_on and NS.last_output_time + NS.frequency == TS.t:
+            NS.last_output_time = TS.t
+            self.stats_io()
+        if Re.enabled and Re.last_restart_time + Re.frequency == TS.t:
+            self.write_restart()
+
+    def stats_io(self):
+        """The statistics output of Simulation3d.io (Simulation3d.pyx:227-256)
+        for the operators on the device path: every reduction runs on the GPU
+        and only the profiles come to the host."""
+        NS = self.StatsIO
+        NS.open_files(self.Pa)
+        NS.write_simulation_time(self.TS.t, self.Pa)
+        self.PV.stats_io(self.Gr, self.Ref, NS, self.Pa)
+        self.DV.stats_io(self.Gr, NS, self.Pa)
+        if hasattr(self.Th, "stats_io"):
+            self.Th.stats_io(self.Gr, self.Ref, self.PV, self.DV, NS, self.Pa)
+        return NS.close_files(self.Pa)
+
+    def write_restart(self):
+        """Dump the restart file for the current time (Simulation3d.pyx:269-283)."""
+        Re = self.Restart
+        Re.last_restart_time = self.TS.t
+        self.Gr.restart(Re)
+        self.Ref.restart(self.Gr, Re)
+        self.PV.restart(self.Gr, Re)
+        self.TS.restart(Re)
+        if hasattr(self.Th, "restart"):
+            self.Th.restart(Re)
+        return Re.write(self.Pa)
 
     timers = None
 
@@ -177,4 +244,5 @@ class Simulation3d:
             self.Pa.root_print(f"T = {self.TS.t} dt = {self.TS.dt} cfl_max = {self.TS.cfl_max} "
                                f"walltime = {self.walltimes[-1]}")
             n += 1
+        self.Restart.cleanup()
         return self

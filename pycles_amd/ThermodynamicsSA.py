@@ -62,6 +62,52 @@ class ThermodynamicsSA:
             self.dims = D.dims_of(Gr)
             self.counters = torch.zeros(4, dtype=torch.int64, device=D.require_gpu())
 
+    def stats_io(self, Gr, RS, PV, DV, NS, Pa):
+        """ThermodynamicsSA.stats_io (ThermodynamicsSA.pyx:256-389): profiles
+        of the entropy potential temperature, the dry potential temperature
+        and the relative humidity, then liquid_stats (:395-492)."""
+        self._prepare(Gr)
+        gw = Gr.dims.gw
+        ref = D.device_ref(RS)
+        base = _capi.ScalarBase(PV.base_ptr("s"), PV.base_ptr("qt")) if PV.precision == D.F32 else None
+        tmp = torch.empty((3,) + tuple(PV.field("s").shape), dtype=PV.values.dtype, device=PV.values.device)
+        _capi.sa_stat_fields(self.dims, ref.c, self._lt, PV.ptr("s"), PV.ptr("qt"), DV.ptr("temperature"),
+                             DV.ptr("qv"), base, PV.precision, tmp[0].data_ptr(), tmp[1].data_ptr(),
+                             tmp[2].data_ptr(), D.stream())
+        for q, name, moments in ((0, "thetas", True), (1, "theta", True), (2, "rh", False)):
+            st = Pa.horizontal_stats(Gr, tmp[q], PV.precision)
+            NS.write_profile(name + "_mean", st[0, gw:-gw], Pa)
+            if moments:
+                NS.write_profile(name + "_mean2", st[1, gw:-gw], Pa)
+                NS.write_profile(name + "_mean3", st[2, gw:-gw], Pa)
+            NS.write_profile(name + "_max", st[3, gw:-gw], Pa)
+            NS.write_profile(name + "_min", st[4, gw:-gw], Pa)
+            if moments:
+                NS.write_ts(name + "_max", np.amax(st[3, gw:-gw]), Pa)
+                NS.write_ts(name + "_min", np.amin(st[4, gw:-gw]), Pa)
+        self.liquid_stats(Gr, RS, PV, DV, NS, Pa)
+
+    def liquid_stats(self, Gr, RS, PV, DV, NS, Pa):
+        """ThermodynamicsSA.liquid_stats (ThermodynamicsSA.pyx:395-492)."""
+        cf, cover, cb, ct, lwp = Pa.cloud_stats(Gr, RS, DV.field("ql"), DV.precision)
+        NS.write_profile("cloud_fraction", cf, Pa)
+        NS.write_ts("cloud_fraction", cover, Pa)
+        NS.write_ts("cloud_base", cb, Pa)
+        NS.write_ts("cloud_top", ct, Pa)
+        NS.write_ts("lwp", lwp, Pa)
+
+    def restart(self, Re):
+        """The learned first guess, when one is in use (no reference
+        counterpart: the reference has no learned part)."""
+        if self.mlp is not None:
+            Re.restart_data["Th"] = {"learned_guess": self.mlp.state_dict()}
+
+    def init_from_restart(self, Re):
+        st = Re.restart_data.get("Th", {}).get("learned_guess")
+        if st is not None:
+            from .LearnedGuess import SaturationGuess
+            self.mlp = SaturationGuess(self.CC, n_pairs=1 << 7, batch=128, epochs=1).load_state_dict(st)
+
     def options(self, wt_overwrite=False):
         return _capi.SAOptions(self.do_qt_clipping, self.max_iter, _capi.LH_VARIABLE,
                                None if self.mlp is None else self.mlp.handle_ptr(),

@@ -78,6 +78,10 @@ class TimeStepping:
         if not self.cfl_max >= 0.0:
             raise RuntimeError(f"CFL_MAX = {self.cfl_max} killing simulation!")
 
+    def restart(self, Re):
+        """TimeStepping.restart (TimeStepping.pyx:268-273)."""
+        Re.restart_data["TS"] = {"t": self.t, "dt": self.dt}
+
     def cfl_time_step(self):
         return min(self.dt_max, self.cfl_limit / (self.cfl_max / self.dt))
 

@@ -83,6 +83,17 @@ cdef extern from "pycles_amd.h":
     int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
                        double dt, double *out, int precision, void *stream)
     int pycles_cfl_read(const double *dev_cfl, double *host_cfl, void *stream)
+    size_t pycles_horizontal_stats_workspace_bytes(const pycles_dims *dims)
+    int pycles_horizontal_stats(const pycles_dims *dims, const void *a, const double *base_a, const void *b,
+                                const double *base_b, const void *mask, int precision, double *out,
+                                void *workspace, void *stream)
+    int pycles_sa_stat_fields(const pycles_dims *dims, const pycles_ref *ref, const pycles_lookup *lt,
+                              const void *s, const void *qt, const void *T, const void *qv,
+                              const pycles_scalar_base *base, int precision, void *thetas, void *theta,
+                              void *rh, void *stream)
+    size_t pycles_cloud_stats_workspace_bytes(const pycles_dims *dims)
+    int pycles_cloud_stats(const pycles_dims *dims, const double *rho0_half, const void *ql, int precision,
+                           double *out, void *workspace, void *stream)
     int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len)
     int pycles_pressure_plan_create(const pycles_dims *dims, const double *rho0_host,
                                     const double *rho0_half_host, int precision,
@@ -329,6 +340,32 @@ def cfl_read(dev_cfl, stream):
     cdef double h = 0.0
     _check(pycles_cfl_read(<const double *> _p(dev_cfl), &h, _p(stream)))
     return h
+
+
+def horizontal_stats_workspace_bytes(Dims dims):
+    return pycles_horizontal_stats_workspace_bytes(&dims.d)
+
+
+def horizontal_stats(Dims dims, a, base_a, b, base_b, mask, int precision, out, workspace, stream):
+    _check(pycles_horizontal_stats(&dims.d, _p(a), <const double *> _p(base_a), _p(b),
+                                   <const double *> _p(base_b), _p(mask), precision, <double *> _p(out),
+                                   _p(workspace), _p(stream)))
+
+
+def sa_stat_fields(Dims dims, Ref ref, Lookup lt, s, qt, T, qv, ScalarBase base, int precision, thetas, theta,
+                   rh, stream):
+    _check(pycles_sa_stat_fields(&dims.d, &ref.r, &lt.t, _p(s), _p(qt), _p(T), _p(qv),
+                                 &base.b if base is not None else NULL, precision, _p(thetas), _p(theta),
+                                 _p(rh), _p(stream)))
+
+
+def cloud_stats_workspace_bytes(Dims dims):
+    return pycles_cloud_stats_workspace_bytes(&dims.d)
+
+
+def cloud_stats(Dims dims, rho0_half, ql, int precision, out, workspace, stream):
+    _check(pycles_cloud_stats(&dims.d, <const double *> _p(rho0_half), _p(ql), precision, <double *> _p(out),
+                              _p(workspace), _p(stream)))
 
 
 def launch_chunks(int kernel, Dims dims):
