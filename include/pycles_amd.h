@@ -119,6 +119,38 @@ int pycles_advection_rk3(const pycles_dims *dims, const pycles_ref *ref, const v
                          void *const *out_scal, void *const copy_vel[3], void *const *copy_scal,
                          int rk_step, double dt, void *stream);
 
+/* ---- kinematics, SGS and diffusion ------------------------------------- */
+/* Replaces Kinematics.update (Kinematics.pyx:41-58: compute_velocity_gradient,
+ * compute_strain_rate, compute_strain_rate_mag, kinematics.h:6-138) and
+ * Smagorinsky.update (SGS.pyx:146-152: smagorinsky_update, sgs.h:4-19) in one
+ * pass: the strain-rate magnitude |S| (0 on the bottom and top ghost
+ * levels, as the reference leaves it), the eddy viscosity
+ * nu = (cs Delta)^2 |S| (times sqrt(max(1 - N^2/(prt |S|^2), 0)) where
+ * N^2 > 0, Delta = (dx dy dz)^(1/3)) and the diffusivity nu/prt, on every
+ * stored point.  vel = u, v, w; bvf = N^2 (the thermodynamics output);
+ * all fields in the given precision. */
+int pycles_sgs_smagorinsky(const pycles_dims *dims, const void *const vel[3], const void *bvf, double cs,
+                           double prt, void *strain_mag, void *visc, void *diff, int precision, void *stream);
+/* flags of pycles_diffusion: PYCLES_DIFF_SCALARS replaces
+ * ScalarDiffusion.update (ScalarDiffusion.pyx:76-119: second_order_diffusion,
+ * scalar_diffusion.h:6-58, and scalar_flux_divergence) for the nscal (1..2)
+ * scalars with the diffusivity diff; PYCLES_DIFF_MOMENTUM the stress part of
+ * MomentumDiffusion.update (MomentumDiffusion.pyx:44-66:
+ * compute_diffusive_flux_m, momentum_diffusion.h:5-88, and
+ * momentum_flux_divergence) with the viscosity visc, and
+ * PYCLES_DIFF_ENTROPY_SOURCE its compute_entropy_source (:90-118) into the
+ * first scalar's tendency from |S|, visc and the temperature.  One pass does
+ * the selected parts in that order; tendencies are accumulated after what
+ * they hold, per variable in the reference's order.  The surface flux is
+ * zero (one GPU owns the bottom); base as in pycles_advection.  gw >= 2. */
+#define PYCLES_DIFF_SCALARS 1
+#define PYCLES_DIFF_MOMENTUM 2
+#define PYCLES_DIFF_ENTROPY_SOURCE 4
+int pycles_diffusion(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
+                     void *const tend_vel[3], int nscal, const void *const *scal, void *const *tend_scal,
+                     const pycles_scalar_base *base, const void *visc, const void *diff, const void *strain_mag,
+                     const void *temperature, int flags, int precision, void *stream);
+
 /* ---- thermodynamics ---------------------------------------------------- */
 /* pv*(T) lookup table (reference Csrc/lookup.h:4-54). */
 typedef struct pycles_lookup {

@@ -615,6 +615,182 @@ void orc_thermo_dry_update(const orc_dims *d, const orc_ref *rs, double *s, doub
 }
 
 /* ------------------------------------------------------------------------ */
+/* Kinematics, SGS and diffusion                                              */
+
+/* Kinematics.update (Kinematics.pyx:41-58): compute_velocity_gradient,
+ * compute_strain_rate, compute_strain_rate_mag (kinematics.h:6-138).
+ * vgrad and strain hold 9 npg arrays, component (vi, d) at (3 vi + d) npg. */
+void orc_kinematics_update(const orc_dims *d, const double *const *vel, double *vgrad, double *strain,
+                           double *smag) {
+    long st[3];
+    strides(d, st);
+    const long n0 = d->nlg[0], n1 = d->nlg[1], n2 = d->nlg[2], npg = d->npg;
+    long count = 0;
+    for (int vi = 0; vi < 3; ++vi)
+        for (int dd = 0; dd < 3; ++dd) {
+            const double *v = vel[vi];
+            double *g = vgrad + count * npg;
+            const long sp1 = st[dd];
+            const double dxi = d->dxi[dd];
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n0 - 1; ++i)
+                for (long j = 0; j < n1 - 1; ++j)
+                    for (long k = 0; k < n2 - 1; ++k) {
+                        const long ijk = i * st[0] + j * st[1] + k;
+                        g[ijk] = (v[ijk + sp1] - v[ijk]) * dxi;
+                    }
+            ++count;
+        }
+    count = 0;
+    for (int v1 = 0; v1 < 3; ++v1)
+        for (int dd = 0; dd < 3; ++dd) {
+            const long s1 = 3 * npg * v1 + npg * dd, s2 = 3 * npg * dd + npg * v1, sh = count * npg;
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < n0 - 1; ++i)
+                for (long j = 0; j < n1 - 1; ++j)
+                    for (long k = 0; k < n2 - 1; ++k) {
+                        const long ijk = i * st[0] + j * st[1] + k;
+                        strain[sh + ijk] = 0.5 * (vgrad[s1 + ijk] + vgrad[s2 + ijk]);
+                    }
+            ++count;
+        }
+    for (long i = 0; i < npg; ++i) smag[i] = 0.0;
+    for (int dd = 0; dd < 3; ++dd) {
+        const long shs = 4 * dd * npg;
+#pragma omp parallel for schedule(static)
+        for (long i = 1; i < n0 - 1; ++i)
+            for (long j = 1; j < n1 - 1; ++j)
+                for (long k = 1; k < n2 - 1; ++k) {
+                    const long ijk = i * st[0] + j * st[1] + k;
+                    const long t = shs + ijk - st[dd];
+                    smag[ijk] += strain[t] * strain[t];
+                }
+    }
+    for (int v1 = 0; v1 < 2; ++v1)
+        for (int dd = v1; dd < 3; ++dd) {
+            const long shs = 3 * npg * v1 + npg * dd;
+#pragma omp parallel for schedule(static)
+            for (long i = 1; i < n0 - 1; ++i)
+                for (long j = 1; j < n1 - 1; ++j)
+                    for (long k = 1; k < n2 - 1; ++k) {
+                        const long ijk = i * st[0] + j * st[1] + k;
+                        const long p1 = shs + ijk, p2 = shs + ijk - st[v1], p3 = shs + ijk - st[dd],
+                                   p4 = shs + ijk - st[dd] - st[v1];
+                        /* the loop over dd starts at v1, so the (0,0) and (1,1)
+                         * pairs are counted again here; at i = 1 the (0,0)
+                         * pair's last point lies before the array, where the
+                         * reference reads whatever precedes it: read as zero
+                         * (no consumer uses |S| there when gw >= 3) */
+                        const double s4 = p4 >= 0 ? strain[p4] : 0.0;
+                        const double si = 0.25 * (strain[p1] + strain[p2] + strain[p3] + s4);
+                        smag[ijk] += 2.0 * si * si;
+                    }
+        }
+    for (long i = 0; i < npg; ++i) smag[i] = sqrt(2.0 * smag[i]);
+}
+
+/* smagorinsky_update, sgs.h:4-19 (Smagorinsky.update without the wall or
+ * iles adjustments, SGS.pyx:146-152). */
+void orc_smagorinsky_update(const orc_dims *d, double *visc, double *diff, const double *bvf, const double *smag,
+                            double cs, double prt) {
+    const double delta = cbrt(d->dx[0] * d->dx[1] * d->dx[2]);
+    for (long i = 0; i < d->npg; ++i) {
+        visc[i] = cs * cs * delta * delta * smag[i];
+        if (bvf[i] > 0.0) {
+            const double fb = sqrt(fmax(1.0 - bvf[i] / (prt * smag[i] * smag[i]), 0.0));
+            visc[i] = visc[i] * fb;
+        }
+        diff[i] = visc[i] / prt;
+    }
+}
+
+/* second_order_diffusion, scalar_diffusion.h:6-58 (factor 1), into flux. */
+static void second_order_diffusion(const orc_dims *d, const double *rho0, const double *rho0_half,
+                                   const double *diffusivity, const double *phi, double *flux, double dx, int dd) {
+    long st[3];
+    strides(d, st);
+    const long gw = d->gw;
+    const double dxi = 1.0 / dx;
+    const double *rho = dd == 2 ? rho0 : rho0_half;
+    const long s = st[dd];
+#pragma omp parallel for schedule(static)
+    for (long i = gw - 1; i < d->nlg[0] - gw; ++i)
+        for (long j = gw - 1; j < d->nlg[1] - gw; ++j)
+            for (long k = gw - 1; k < d->nlg[2] - gw; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                flux[ijk] = -1.0 * orc_interp_2(diffusivity[ijk], diffusivity[ijk + s]) * (phi[ijk + s] - phi[ijk]) *
+                            rho[k] * dxi;
+            }
+    /* the surface flux is exactly zero (one rank owns the bottom) */
+    for (long i = gw - 1; i < d->nlg[0] - gw; ++i)
+        for (long j = gw - 1; j < d->nlg[1] - gw; ++j) flux[i * st[0] + j * st[1] + gw - 1] = 0.0;
+}
+
+/* ScalarDiffusion.update (ScalarDiffusion.pyx:76-119) for nsc scalars
+ * without the qt entropy source (its default, :31-36). */
+void orc_scalar_diffusion_update(const orc_dims *d, const orc_ref *rs, const double *diffusivity, int nsc,
+                                 const double *const *phi, double *const *tend, double *flux) {
+    for (int sc = 0; sc < nsc; ++sc)
+        for (int dd = 0; dd < 3; ++dd) {
+            second_order_diffusion(d, rs->rho0, rs->rho0_half, diffusivity, phi[sc], flux, d->dx[dd], dd);
+            orc_scalar_flux_divergence(d, rs->alpha0, rs->alpha0_half, flux, tend[sc], dd);
+        }
+}
+
+/* compute_diffusive_flux_m, momentum_diffusion.h:5-88. */
+static void diffusive_flux_m(const orc_dims *d, const double *strain, const double *visc, double *flux,
+                             const double *rho0, const double *rho0_half, int i1, int i2) {
+    long st[3];
+    strides(d, st);
+    const long gw = d->gw;
+    const long s1 = st[i1], s2 = st[i2];
+#pragma omp parallel for schedule(static)
+    for (long i = gw - 1; i < d->nlg[0] - gw; ++i)
+        for (long j = gw - 1; j < d->nlg[1] - gw; ++j)
+            for (long k = gw - 1; k < d->nlg[2] - gw; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                if (i1 != 2 && i2 != 2) {
+                    if (i1 == i2) {
+                        flux[ijk] = -2.0 * strain[ijk] * visc[ijk + s1] * rho0_half[k];
+                    } else {
+                        const double vi = 0.25 * (visc[ijk] + visc[ijk + s1] + visc[ijk + s2] + visc[ijk + s1 + s2]);
+                        flux[ijk] = -2.0 * strain[ijk] * vi * rho0_half[k];
+                    }
+                } else if (i1 == 2 && i2 == 2) {
+                    flux[ijk] = -2.0 * strain[ijk] * visc[ijk + s1] * rho0_half[k + 1];
+                } else {
+                    const double vi = 0.25 * (visc[ijk] + visc[ijk + s1] + visc[ijk + s2] + visc[ijk + s1 + s2]);
+                    flux[ijk] = -2.0 * strain[ijk] * vi * rho0[k];
+                }
+            }
+    for (long i = gw - 1; i < d->nlg[0] - gw; ++i)
+        for (long j = gw - 1; j < d->nlg[1] - gw; ++j) flux[i * st[0] + j * st[1] + gw - 1] = 0.0;
+}
+
+/* MomentumDiffusion.update (MomentumDiffusion.pyx:44-66): the nine stress
+ * fluxes and their divergence, then the entropy source of the dissipation,
+ * compute_entropy_source (momentum_diffusion.h:90-118), into s_tend. */
+void orc_momentum_diffusion_update(const orc_dims *d, const orc_ref *rs, const double *strain,
+                                   const double *visc, double *const *tend, double *flux, const double *smag,
+                                   const double *T, double *s_tend) {
+    long st[3];
+    strides(d, st);
+    for (int i1 = 0; i1 < 3; ++i1)
+        for (int i2 = 0; i2 < 3; ++i2) {
+            diffusive_flux_m(d, strain + (long)(3 * i1 + i2) * d->npg, visc, flux, rs->rho0, rs->rho0_half, i1, i2);
+            momentum_flux_divergence(d, rs, flux, tend[i1], i1, i2);
+        }
+    const long gw = d->gw;
+#pragma omp parallel for schedule(static)
+    for (long i = gw; i < d->nlg[0] - gw; ++i)
+        for (long j = gw; j < d->nlg[1] - gw; ++j)
+            for (long k = gw; k < d->nlg[2] - gw; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                s_tend[ijk] += smag[ijk] * smag[ijk] * visc[ijk] / T[ijk];
+            }
+}
+
+/* ------------------------------------------------------------------------ */
 /* State, boundaries, time stepping                                          */
 
 /* One variable: periodic x then periodic y (build_buffer/buffer_to_values

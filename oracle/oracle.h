@@ -126,6 +126,16 @@ void orc_thermo_dry_update(const orc_dims *d, const orc_ref *rs, double *s, doub
 /* ---- state, boundaries, time stepping ---- */
 /* PrognosticVariables.update_all_bcs for one variable on one rank
  * (PrognosticVariables.pyx:183-233 + prognostic_variables.h:205-270). */
+/* ---- kinematics, SGS and diffusion ---- */
+void orc_kinematics_update(const orc_dims *d, const double *const *vel, double *vgrad, double *strain,
+                           double *smag);
+void orc_smagorinsky_update(const orc_dims *d, double *visc, double *diff, const double *bvf, const double *smag,
+                            double cs, double prt);
+void orc_scalar_diffusion_update(const orc_dims *d, const orc_ref *rs, const double *diffusivity, int nsc,
+                                 const double *const *phi, double *const *tend, double *flux);
+void orc_momentum_diffusion_update(const orc_dims *d, const orc_ref *rs, const double *strain,
+                                   const double *visc, double *const *tend, double *flux, const double *smag,
+                                   const double *T, double *s_tend);
 void orc_update_bcs(const orc_dims *d, double *values, double bc_factor);
 /* TimeStepping.update_third (TimeStepping.pyx:131-151) over n = npg*nv points. */
 void orc_rk3_update(long n, int rk_step, double dt, double *values, double *tend,

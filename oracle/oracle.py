@@ -87,6 +87,12 @@ def lib():
         L.orc_tdma_solve.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp]
         L.orc_fft.argtypes = [C.c_long, _dp, _dp, C.c_int]
         L.orc_update_bcs.argtypes = [C.POINTER(OrcDims), _dp, d]
+        L.orc_kinematics_update.argtypes = [C.POINTER(OrcDims), C.POINTER(_dp), _dp, _dp, _dp]
+        L.orc_smagorinsky_update.argtypes = [C.POINTER(OrcDims), _dp, _dp, _dp, _dp, d, d]
+        L.orc_scalar_diffusion_update.argtypes = [C.POINTER(OrcDims), C.POINTER(OrcRef), _dp, C.c_int,
+                                                  C.POINTER(_dp), C.POINTER(_dp), _dp]
+        L.orc_momentum_diffusion_update.argtypes = [C.POINTER(OrcDims), C.POINTER(OrcRef), _dp, _dp,
+                                                    C.POINTER(_dp), _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -177,7 +183,7 @@ class OracleSimulation:
 
     def __init__(self, dims: OrcDims, ref: Ref, moist: bool, lookup: Lookup | None,
                  scalar_order=5, momentum_order=5, dt=1.0, dt_max=None, cfl_limit=1e6,
-                 t_max=1e30, max_iter=0):
+                 t_max=1e30, max_iter=0, sgs=None):
         self.d = dims
         self.ref = ref
         self.moist = moist
@@ -202,6 +208,14 @@ class OracleSimulation:
         self.max_iter = max_iter
         self.cfl_max = 0.0
         self.sa_stats = OrcSAStats()
+        # Smagorinsky SGS with scalar and momentum diffusion: {"cs", "prt"}
+        self.sgs = sgs
+        if sgs is not None:
+            self.dv["viscosity"] = np.zeros(self.shape)
+            self.dv["diffusivity"] = np.zeros(self.shape)
+            self.vgrad = np.zeros((9,) + self.shape)
+            self.strain = np.zeros((9,) + self.shape)
+            self.smag = np.zeros(self.shape)
 
     # -- accessors ---------------------------------------------------------
     def pv(self, name):
@@ -246,6 +260,30 @@ class OracleSimulation:
         lib().orc_momentum_advection_update(C.byref(self.d), C.byref(self.ref.c), vel, tend, self.mo,
                                             ptr(self.flux))
 
+    def kinematics_update(self):
+        """Kinematics.update (Kinematics.pyx:41-58)."""
+        vel = (_dp * 3)(*[ptr(self.pv(n)) for n in ("u", "v", "w")])
+        lib().orc_kinematics_update(C.byref(self.d), vel, ptr(self.vgrad), ptr(self.strain), ptr(self.smag))
+
+    def sgs_update(self):
+        """Smagorinsky.update (SGS.pyx:146-152), no wall or iles adjustment."""
+        lib().orc_smagorinsky_update(C.byref(self.d), ptr(self.dv["viscosity"]), ptr(self.dv["diffusivity"]),
+                                     ptr(self.dv["buoyancy_frequency"]), ptr(self.smag), self.sgs["cs"],
+                                     self.sgs["prt"])
+
+    def scalar_diffusion_update(self):
+        names = [n for n in self.pv_names if n in ("s", "qt")]
+        phi = (_dp * len(names))(*[ptr(self.pv(n)) for n in names])
+        tend = (_dp * len(names))(*[ptr(self.pt(n)) for n in names])
+        lib().orc_scalar_diffusion_update(C.byref(self.d), C.byref(self.ref.c), ptr(self.dv["diffusivity"]),
+                                          len(names), phi, tend, ptr(self.flux))
+
+    def momentum_diffusion_update(self):
+        tend = (_dp * 3)(*[ptr(self.pt(n)) for n in ("u", "v", "w")])
+        lib().orc_momentum_diffusion_update(C.byref(self.d), C.byref(self.ref.c), ptr(self.strain),
+                                            ptr(self.dv["viscosity"]), tend, ptr(self.flux), ptr(self.smag),
+                                            ptr(self.dv["temperature"]), ptr(self.pt("s")))
+
     def rk_update(self, rk_step):
         lib().orc_rk3_update(self.values.size, rk_step, self.dt, ptr(self.values), ptr(self.tend),
                              ptr(self.copy))
@@ -268,9 +306,17 @@ class OracleSimulation:
             self.dt = self.t_max - self.t
 
     def stage(self, rk_step):
+        """Simulation3d.run's stage (Simulation3d.pyx:158-177) restricted to
+        the operators restated here."""
+        if self.sgs is not None:
+            self.kinematics_update()
         self.thermo_update()
         self.scalar_advection_update()
         self.momentum_advection_update()
+        if self.sgs is not None:
+            self.sgs_update()
+            self.scalar_diffusion_update()
+            self.momentum_diffusion_update()
         self.rk_update(rk_step)
         self.update_all_bcs()
         self.pressure_update()

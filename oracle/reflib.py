@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY.  ``make -C oracle ref`` compiles, straight from the
 reference tree, the headers that build without generated code:
 ``Csrc/momentum_advection.h`` (with the interpolation and flux-divergence
-headers it includes) and ``Csrc/prognostic_variables.h``.  Tests use it to
+headers it includes), ``Csrc/prognostic_variables.h``, ``Csrc/kinematics.h``
+and ``Csrc/momentum_diffusion.h``.  Tests use it to
 check the oracle bit for bit; it is never shipped and never measured.  It
 exists only where the reference tree is present, so tests that use it skip
 elsewhere.
@@ -21,8 +22,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_ref", "libpycles_ref.so")
+_KIN_PATH = os.path.join(_HERE, "_ref", "libpycles_ref_kin.so")
 REF_SRC = "/root/reference/Csrc"
 _lib = None
+_kin = None
 _dp = C.POINTER(C.c_double)
 
 
@@ -63,8 +66,25 @@ def lib():
         L.build_buffer.argtypes = [C.c_long, C.c_long, C.c_long, C.POINTER(DimStruct), _dp, _dp]
         L.buffer_to_values.argtypes = [C.c_ssize_t, C.c_ssize_t, C.POINTER(DimStruct), _dp, _dp]
         L.set_bcs.argtypes = [C.c_ssize_t, C.c_ssize_t, d, C.POINTER(DimStruct), _dp]
+        L.compute_diffusive_flux_m.argtypes = [C.POINTER(DimStruct), _dp, _dp, _dp, _dp, _dp, C.c_ssize_t,
+                                               C.c_ssize_t]
+        L.momentum_flux_divergence.argtypes = [C.POINTER(DimStruct), _dp, _dp, _dp, _dp, C.c_ssize_t, C.c_ssize_t]
+        L.compute_entropy_source.argtypes = [C.POINTER(DimStruct), _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
+
+
+def kin():
+    global _kin
+    if _kin is None:
+        if not os.path.exists(_KIN_PATH):
+            build()
+        K = C.CDLL(_KIN_PATH)
+        K.compute_velocity_gradient.argtypes = [C.POINTER(DimStruct), _dp, _dp, C.c_ssize_t]
+        K.compute_strain_rate.argtypes = [C.POINTER(DimStruct), _dp, _dp]
+        K.compute_strain_rate_mag.argtypes = [C.POINTER(DimStruct), _dp, _dp]
+        _kin = K
+    return _kin
 
 
 def ptr(a: np.ndarray):
@@ -137,3 +157,46 @@ def update_all_bcs(dims: DimStruct, values: np.ndarray, bc_factor: float):
                 L.buffer_to_values(d, s, C.byref(dims), ptr(values), ptr(buf))
             else:
                 L.set_bcs(d, s, bc_factor, C.byref(dims), ptr(values))
+
+
+def kinematics(dims: DimStruct, vel):
+    """Kinematics.update (Kinematics.pyx:41-58) with the reference's
+    compute_velocity_gradient / compute_strain_rate / compute_strain_rate_mag:
+    returns (vgrad[9 npg], strain[9 npg], strain_mag[npg])."""
+    L = kin()
+    npg = dims.npg
+    vgrad = np.zeros(9 * npg)
+    # compute_strain_rate_mag reads up to one x-plane before the strain array
+    # (i = 1, the (0,0) pair at ijk - 2 istride): give it zeros to read
+    pad = dims.nlg[1] * dims.nlg[2]
+    sbuf = np.zeros(pad + 9 * npg)
+    strain = sbuf[pad:]
+    smag = np.zeros(npg)
+    count = 0
+    for vi in range(3):
+        for d in range(3):
+            g = vgrad[count * npg:(count + 1) * npg]
+            L.compute_velocity_gradient(C.byref(dims), ptr(vel[vi].reshape(-1)), ptr(g), d)
+            count += 1
+    L.compute_strain_rate(C.byref(dims), ptr(vgrad), ptr(strain))
+    L.compute_strain_rate_mag(C.byref(dims), ptr(strain), ptr(smag))
+    return vgrad, strain, smag
+
+
+def momentum_diffusion(dims: DimStruct, rho0, rho0_half, alpha0, alpha0_half, strain, visc, tend, smag, T,
+                       s_tend):
+    """MomentumDiffusion.update (MomentumDiffusion.pyx:44-66): the nine
+    stress fluxes, their divergence into tend[3] and the entropy source into
+    s_tend, with the reference's own functions."""
+    L = lib()
+    npg = dims.npg
+    flux = np.zeros(npg)
+    count = 0
+    for i1 in range(3):
+        for i2 in range(3):
+            L.compute_diffusive_flux_m(C.byref(dims), ptr(strain[count * npg:(count + 1) * npg]), ptr(visc),
+                                       ptr(flux), ptr(rho0), ptr(rho0_half), i1, i2)
+            L.momentum_flux_divergence(C.byref(dims), ptr(alpha0), ptr(alpha0_half), ptr(flux), ptr(tend[i1]),
+                                       i1, i2)
+            count += 1
+    L.compute_entropy_source(C.byref(dims), ptr(visc), ptr(smag), ptr(T), ptr(s_tend))

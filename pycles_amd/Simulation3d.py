@@ -4,11 +4,13 @@
 (``Simulation3d.pyx:40-138``) restricted to the hot path; ``run()`` follows
 ``:140-189``: one boundary fill and one thermodynamics update before the loop
 (the latter adds buoyancy to the w tendency, which the first stage keeps), then
-for each RK stage Th.update, SA.update, MA.update, TS.update (with the z
-boundary fill), Pr.update and, on the last stage, TS.adjust_timestep.  The
-operators outside the hot path (kinematics, SGS, diffusion, surface, forcing,
-radiation, damping, tracers, microphysics processes, IO) are not part of
-this build and contribute nothing, as with a namelist that disables them.
+for each RK stage Ke.update, Th.update, SA.update, MA.update, SGS.update,
+SD.update and MD.update (with an sgs section in the namelist), TS.update (with
+the z boundary fill), Pr.update and, on the last stage, TS.adjust_timestep and
+the restart and statistics output.  The operators outside this path
+(surface, forcing, radiation, damping, tracers, microphysics processes) are
+not part of this build and contribute nothing, as with a namelist that
+disables them.
 """
 from __future__ import annotations
 
@@ -52,6 +54,15 @@ class Simulation3d:
         self.MA = MomentumAdvection(namelist, self.Pa)
         self.Th = Th if Th is not None else ThermodynamicsFactory(namelist, None, self.LH, self.Pa)
         self.Ref = Ref if Ref is not None else ReferenceState(self.Gr)
+        # kinematics, SGS closure and diffusion when the namelist has an
+        # sgs section (Simulation3d.pyx:44-53)
+        self.Ke = self.SGS = self.SD = self.MD = None
+        if "sgs" in namelist:
+            from .SGS import SGS, Kinematics, MomentumDiffusion, ScalarDiffusion
+            self.Ke = Kinematics()
+            self.SGS = SGS(namelist)
+            self.SD = ScalarDiffusion(namelist, self.LH, self.DV, self.Pa)
+            self.MD = MomentumDiffusion(self.DV, self.Pa)
         self.TS = TimeStepping()
         self.StatsIO = NetCDFIO_Stats().initialize(namelist, None, None)
         self.Restart = Restart(namelist, self.Pa)
@@ -66,6 +77,9 @@ class Simulation3d:
         self.PV.initialize(self.Gr, self.StatsIO, self.Pa, precision=precision)
         self.SA.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
         self.MA.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
+        if self.SGS is not None:
+            self.Ke.initialize(self.Gr, self.StatsIO, self.Pa, precision=precision)
+            self.SGS.initialize(self.Gr, self.PV, self.StatsIO, self.Pa)
         self.TS.initialize(namelist, self.PV, self.Pa)
         if self.Restart.is_restart_run:
             # Simulation3d.pyx:103-121: state, time and reference from the file
@@ -126,10 +140,14 @@ class Simulation3d:
         same bits as zeroing and accumulating, with ten fewer field passes."""
         self.TS.rk_step = rk_step
         fast = self.write_tendencies
+        if self.Ke is not None:
+            self.Ke.update(self.Gr, self.PV)
         self._mark("thermo")
         self.Th.update(self.Gr, self.Ref, self.PV, self.DV, wt_overwrite=fast and not self._wt_pending)
         self._wt_pending = False
-        fused_rk = self.fuse_rk and self.fuse_advection and fast
+        # with SGS and diffusion the RK update runs after them (their
+        # tendencies add to the advection's first, as in the reference)
+        fused_rk = self.fuse_rk and self.fuse_advection and fast and self.SGS is None
         if fused_rk:
             if self.timers is not None:
                 from ._device import _capi
@@ -159,6 +177,12 @@ class Simulation3d:
             self._mark("advection")
             self.SA.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa, tend_zero=fast)
             self.MA.update(self.Gr, self.Ref, self.PV, self.Pa, tend_zero=fast)
+        if self.SGS is not None:
+            from .SGS import diffuse
+            self._mark("sgs")
+            self.SGS.update(self.Gr, self.DV, self.PV, self.Ke, None, self.Pa)
+            self._mark("diffusion")
+            diffuse(self.SD, self.MD, self.Gr, self.Ref, self.PV, self.DV, self.Ke)
         if not fused_rk:
             self._mark("rk3")
             self.TS.update(self.Gr, self.PV, self.Pa, zero_tendencies=not fast)

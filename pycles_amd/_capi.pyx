@@ -51,6 +51,9 @@ cdef extern from "pycles_amd.h":
     int PYCLES_ADV_PASS_X
     int PYCLES_ADV_PASS_Y
     int PYCLES_ADV_PASS_Z
+    int PYCLES_DIFF_SCALARS
+    int PYCLES_DIFF_MOMENTUM
+    int PYCLES_DIFF_ENTROPY_SOURCE
 
     const char *pycles_last_error()
     const char *pycles_version()
@@ -83,6 +86,12 @@ cdef extern from "pycles_amd.h":
     int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
                        double dt, double *out, int precision, void *stream)
     int pycles_cfl_read(const double *dev_cfl, double *host_cfl, void *stream)
+    int pycles_sgs_smagorinsky(const pycles_dims *dims, const void *const *vel, const void *bvf, double cs,
+                               double prt, void *strain_mag, void *visc, void *diff, int precision, void *stream)
+    int pycles_diffusion(const pycles_dims *dims, const pycles_ref *ref, const void *const *vel,
+                         void *const *tend_vel, int nscal, const void *const *scal, void *const *tend_scal,
+                         const pycles_scalar_base *base, const void *visc, const void *diff,
+                         const void *strain_mag, const void *temperature, int flags, int precision, void *stream)
     size_t pycles_horizontal_stats_workspace_bytes(const pycles_dims *dims)
     int pycles_horizontal_stats(const pycles_dims *dims, const void *a, const double *base_a, const void *b,
                                 const double *base_b, const void *mask, int precision, double *out,
@@ -134,6 +143,9 @@ ADV_TEND_ZERO = PYCLES_ADV_TEND_ZERO
 ADV_PASS_X = PYCLES_ADV_PASS_X
 ADV_PASS_Y = PYCLES_ADV_PASS_Y
 ADV_PASS_Z = PYCLES_ADV_PASS_Z
+DIFF_SCALARS = PYCLES_DIFF_SCALARS
+DIFF_MOMENTUM = PYCLES_DIFF_MOMENTUM
+DIFF_ENTROPY_SOURCE = PYCLES_DIFF_ENTROPY_SOURCE
 
 
 class PyclesError(RuntimeError):
@@ -366,6 +378,32 @@ def cloud_stats_workspace_bytes(Dims dims):
 def cloud_stats(Dims dims, rho0_half, ql, int precision, out, workspace, stream):
     _check(pycles_cloud_stats(&dims.d, <const double *> _p(rho0_half), _p(ql), precision, <double *> _p(out),
                               _p(workspace), _p(stream)))
+
+
+def sgs_smagorinsky(Dims dims, vel, bvf, double cs, double prt, strain_mag, visc, diff, int precision, stream):
+    cdef const void *v3[3]
+    for i in range(3):
+        v3[i] = _p(vel[i])
+    _check(pycles_sgs_smagorinsky(&dims.d, v3, _p(bvf), cs, prt, _p(strain_mag), _p(visc), _p(diff), precision,
+                                  _p(stream)))
+
+
+def diffusion(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase base, visc, diff, strain_mag,
+              temperature, int flags, int precision, stream):
+    cdef const void *v3[3]
+    cdef void *t3[3]
+    cdef const void *sc[2]
+    cdef void *ts[2]
+    cdef int ns = len(scal)
+    for i in range(3):
+        v3[i] = _p(vel[i])
+        t3[i] = _p(tend_vel[i]) if tend_vel is not None else NULL
+    for i in range(ns):
+        sc[i] = _p(scal[i])
+        ts[i] = _p(tend_scal[i])
+    _check(pycles_diffusion(&dims.d, &ref.r, v3, t3 if tend_vel is not None else NULL, ns, sc, ts,
+                            &base.b if base is not None else NULL, _p(visc), _p(diff), _p(strain_mag),
+                            _p(temperature), flags, precision, _p(stream)))
 
 
 def launch_chunks(int kernel, Dims dims):

@@ -1,0 +1,118 @@
+"""Kinematics, Smagorinsky SGS and diffusion on the GPU against the oracle
+(which is bitwise equal to the reference's own kinematics.h and
+momentum_diffusion.h, and pinned by mathematics for sgs.h and
+scalar_diffusion.h).
+
+FP64 tolerances: the SGS kernel and the diffusion kernel use + - * / and
+sqrt (correctly rounded on both sides) and a cbrt evaluated on the host, so
+|S|, nu, D and the diffusive tendencies are compared bitwise; whole steps
+include the thermodynamics' transcendentals and are compared at 1e-12."""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from tests.helpers import host_case, oracle_sim, rel_err
+
+pytestmark = pytest.mark.gpu
+
+SGS_NL = {"scheme": "Smagorinsky", "Smagorinsky": {"cs": 0.17, "prt": 1.0 / 3.0}}
+SGS_OR = {"cs": 0.17, "prt": 1.0 / 3.0}
+
+
+def _case(nx, ny, nz, seed, precision="f64", gw=3, order=(5, 5)):
+    from tests.test_gpu_parity import _saturating, _swirl
+    nl, Gr, RS, Th, fields = host_case(nx, ny, nz, gw=gw, seed=seed, order=order)
+    fields = _saturating(Gr, _swirl(Gr, fields, amp=3.0))
+    nl = copy.deepcopy(nl)
+    nl["sgs"] = copy.deepcopy(SGS_NL)
+    nl["mi355x"]["precision"] = precision
+    return nl, Gr, RS, Th, fields
+
+
+def _sim(nl, Gr, RS, Th, fields):
+    from pycles_amd.Simulation3d import Simulation3d
+    return Simulation3d().initialize(nl, Gr=Gr, Ref=RS, Th=Th, fields=fields)
+
+
+def test_sgs_and_diffusion_operators_bitwise(gpu):
+    nl, Gr, RS, Th, fields = _case(24, 16, 20, seed=51)
+    sim = _sim(nl, Gr, RS, Th, fields)
+    sim.Th.max_iter = 0
+    sim.Th.update(sim.Gr, sim.Ref, sim.PV, sim.DV)
+    osim = oracle_sim(Gr, RS, Th, fields, True, sgs=SGS_OR)
+    osim.thermo_update()
+    # the GPU's N^2 where the oracle's is defined, so both closures see the same input
+    gw = Gr.dims.gw
+    sim.Ke.update(sim.Gr, sim.PV)
+    sim.SGS.update(sim.Gr, sim.DV, sim.PV, sim.Ke)
+    osim.dv["buoyancy_frequency"][...] = sim.DV.get_variable_array("buoyancy_frequency", Gr)
+    osim.kinematics_update()
+    osim.sgs_update()
+    torch.cuda.synchronize()
+    lv = slice(1, Gr.dims.nzg - 1)
+    smag = sim.Ke.strain_rate_mag.to(torch.float64).cpu().numpy()
+    assert np.array_equal(smag[lv], Gr.to_device_layout(osim.smag)[lv])
+    for n in ("viscosity", "diffusivity"):
+        assert np.array_equal(sim.DV.get_device_layout(n)[lv], Gr.to_device_layout(osim.dv[n])[lv]), n
+    # diffusion on top of the same (random) tendencies
+    rng = np.random.default_rng(5)
+    for n in osim.pv_names:
+        t = rng.normal(size=osim.pt(n).shape)
+        osim.pt(n)[...] = t
+        sim.PV.tend(n).copy_(torch.from_numpy(Gr.to_device_layout(t)))
+    osim.dv["viscosity"][...] = sim.DV.get_variable_array("viscosity", Gr)
+    osim.dv["diffusivity"][...] = sim.DV.get_variable_array("diffusivity", Gr)
+    osim.dv["temperature"][...] = sim.DV.get_variable_array("temperature", Gr)
+    osim.smag[...] = Gr.from_device_layout(smag)
+    from pycles_amd.SGS import diffuse
+    diffuse(sim.SD, sim.MD, sim.Gr, sim.Ref, sim.PV, sim.DV, sim.Ke)
+    osim.scalar_diffusion_update()
+    osim.momentum_diffusion_update()
+    torch.cuda.synchronize()
+    for n in osim.pv_names:
+        g = sim.PV.get_tendency_array(n, Gr)[gw:-gw, gw:-gw, gw:-gw]
+        o = osim.pt(n)[gw:-gw, gw:-gw, gw:-gw]
+        assert np.array_equal(g, o), f"{n}: max diff {np.abs(g - o).max()} (rel {rel_err(g, o)})"
+
+
+def test_steps_with_sgs_match_the_oracle(gpu):
+    nl, Gr, RS, Th, fields = _case(32, 16, 24, seed=52)
+    sim = _sim(nl, Gr, RS, Th, fields)
+    sim.Th.max_iter = 0
+    sim.run(max_steps=2)
+    torch.cuda.synchronize()
+    osim = oracle_sim(Gr, RS, Th, fields, True, sgs=SGS_OR)
+    osim.begin_run()
+    osim.step()
+    osim.step()
+    errs = {n: rel_err(sim.PV.get_device_layout(n), Gr.to_device_layout(osim.pv(n))) for n in osim.pv_names}
+    for n in ("temperature", "viscosity", "dynamic_pressure"):
+        errs[n] = rel_err(sim.DV.get_device_layout(n), Gr.to_device_layout(osim.dv[n]))
+    print("2 FP64 steps with Smagorinsky SGS and diffusion vs oracle", errs)
+    assert max(errs.values()) <= 1e-12, errs
+    # the closure is active: the viscosity is not negligible
+    assert float(sim.DV.field("viscosity").abs().max()) > 1e-2
+
+
+def test_fp32_with_sgs_tracks_fp64(gpu):
+    out = {}
+    for precision in ("f64", "f32"):
+        nl, Gr, RS, Th, fields = _case(32, 32, 32, seed=53, precision=precision)
+        sim = _sim(nl, Gr, RS, Th, fields)
+        sim.run(max_steps=3)
+        torch.cuda.synchronize()
+        out[precision] = sim
+    gw = out["f64"].Gr.dims.gw
+    for n in ("s", "qt"):
+        a = out["f32"].PV.get_device_layout(n)[gw:-gw]
+        b = out["f64"].PV.get_device_layout(n)[gw:-gw]
+        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-5, n
+    a = out["f32"].DV.get_device_layout("temperature")[gw:-gw]
+    b = out["f64"].DV.get_device_layout("temperature")[gw:-gw]
+    assert np.max(np.abs(a - b) / b) < 1e-5
+    for n in ("u", "v", "w"):
+        a = out["f32"].PV.get_device_layout(n)
+        b = out["f64"].PV.get_device_layout(n)
+        assert np.max(np.abs(a - b)) < 1e-3 * max(1.0, np.max(np.abs(b))), n
