@@ -27,7 +27,11 @@ evaluates them at ``Gr.zl_half``, which Grid.pyx:166-167 sets to full levels).
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
+import scipy.fft
 
 from . import parameters as P
 from . import thermo_host as H
@@ -44,20 +48,58 @@ def _splitmix64(x):
     return z ^ (z >> np.uint64(31))
 
 
+_CHUNK = 1 << 21
+
+
+def _workers():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _chunked(n, fn):
+    """Fill an array of n values chunk by chunk on a thread pool (numpy's
+    element-wise operations release the GIL); the values do not depend on
+    the chunking."""
+    out = np.empty(n)
+    bounds = [(a, min(a + _CHUNK, n)) for a in range(0, n, _CHUNK)]
+
+    def run(b):
+        out[b[0]:b[1]] = fn(b[0], b[1])
+
+    if len(bounds) <= 1:
+        for b in bounds:
+            run(b)
+    else:
+        with ThreadPoolExecutor(max_workers=_workers()) as ex:
+            list(ex.map(run, bounds))
+    return out
+
+
+def _uniform_range(key, a, b):
+    with np.errstate(over="ignore"):
+        z = _splitmix64(np.arange(a, b, dtype=np.uint64) ^ key)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _key(seed, field):
+    with np.errstate(over="ignore"):
+        return _splitmix64(np.uint64(seed) * _G ^ (np.uint64(field) << np.uint64(56)))
+
+
 def uniform(seed: int, field: int, shape):
     """U[0,1) per cell of an interior block of the given (nz, ny, nx) shape."""
-    n = int(np.prod(shape))
-    with np.errstate(over="ignore"):
-        key = _splitmix64(np.uint64(seed) * _G ^ (np.uint64(field) << np.uint64(56)))
-        idx = np.arange(n, dtype=np.uint64)
-        z = _splitmix64(idx ^ key)
-    return ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)).reshape(shape)
+    key = _key(seed, field)
+    return _chunked(int(np.prod(shape)), lambda a, b: _uniform_range(key, a, b)).reshape(shape)
 
 
 def normal(seed: int, field: int, shape):
-    u1 = 1.0 - uniform(seed, 2 * field, shape)
-    u2 = uniform(seed, 2 * field + 1, shape)
-    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    k1, k2 = _key(seed, 2 * field), _key(seed, 2 * field + 1)
+
+    def box_muller(a, b):
+        u1 = 1.0 - _uniform_range(k1, a, b)
+        u2 = _uniform_range(k2, a, b)
+        return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+    return _chunked(int(np.prod(shape)), box_muller).reshape(shape)
 
 
 def fill_z_ghosts(f: np.ndarray, gw: int, bc: float):
@@ -145,7 +187,7 @@ def project_initial(Gr, RS, u, v, w):
     div = (ui / ah - np.roll(ui, 1, axis=2) / ah) / dx
     div = div + (vi / ah - np.roll(vi, 1, axis=1) / ah) / dy
     div = div + (w[sl] / a0 - w[gw - 1:gw - 1 + nz] / a0m) / dz
-    spec = np.fft.fft2(div, axes=(1, 2))
+    spec = scipy.fft.fft2(div, axes=(1, 2), workers=_workers())
     kx = np.arange(d.n[0]); kx = np.where(kx <= d.n[0] // 2, kx, kx - d.n[0])
     ky = np.arange(d.n[1]); ky = np.where(ky <= d.n[1] // 2, ky, ky - d.n[1])
     kx2 = (2.0 * np.cos(2.0 * np.pi / d.n[0] * kx) - 2.0) / dx / dx
@@ -160,21 +202,29 @@ def project_initial(Gr, RS, u, v, w):
     dd[0] = rho0[gw] * dzi2
     kk0 = kk.copy()
     kk0[0, 0] = 1.0  # placeholder; the (0,0) mode is zeroed below
-    scr = np.empty((nz,) + kk.shape)
     x = np.empty_like(spec)
-    b = rho0h[gw] * kk0 - dd[0]
-    scr[0] = c[0] / b
-    x[0] = spec[0] / b
-    for k in range(1, nz):
-        b = rho0h[k + gw] * kk0 - dd[k]
-        m = 1.0 / (b - a[k] * scr[k - 1])
-        scr[k] = c[k] * m
-        x[k] = (spec[k] - a[k] * x[k - 1]) * m
-    for k in range(nz - 2, -1, -1):
-        x[k] = x[k] - scr[k] * x[k + 1]
+
+    def tdma_rows(j0, j1):
+        # the tridiagonal solve of the columns of wavenumber rows [j0, j1)
+        kr = kk0[j0:j1]
+        scr = np.empty((nz,) + kr.shape)
+        b = rho0h[gw] * kr - dd[0]
+        scr[0] = c[0] / b
+        x[0, j0:j1] = spec[0, j0:j1] / b
+        for k in range(1, nz):
+            b = rho0h[k + gw] * kr - dd[k]
+            m = 1.0 / (b - a[k] * scr[k - 1])
+            scr[k] = c[k] * m
+            x[k, j0:j1] = (spec[k, j0:j1] - a[k] * x[k - 1, j0:j1]) * m
+        for k in range(nz - 2, -1, -1):
+            x[k, j0:j1] = x[k, j0:j1] - scr[k] * x[k + 1, j0:j1]
+
+    step = max(1, d.n[1] // _workers())
+    with ThreadPoolExecutor(max_workers=_workers()) as ex:
+        list(ex.map(lambda j: tdma_rows(j, min(j + step, d.n[1])), range(0, d.n[1], step)))
     x[:, 0, 0] = 0.0
     p = np.zeros_like(u)
-    p[sl] = np.real(np.fft.ifft2(x, axes=(1, 2)))
+    p[sl] = np.real(scipy.fft.ifft2(x, axes=(1, 2), workers=_workers()))
     fill_z_ghosts(p, gw, 1.0)
     u = u - (np.roll(p, -1, axis=2) - p) / dx
     v = v - (np.roll(p, -1, axis=1) - p) / dy
