@@ -53,6 +53,42 @@ def test_reference_state_is_hydrostatic_and_isentropic(moist):
         assert RS.alpha0_half[gw + k] == pytest.approx(O.lib().orc_alpha(p0, T, RS.qtg, qv), rel=1e-10)
 
 
+@pytest.mark.parametrize("qtg", [0.0, 0.004])
+def test_reference_state_against_the_adiabat_in_closed_form(qtg):
+    """With a constant entropy and an unsaturated constant qt the reference
+    column is an adiabat in closed form: s(p, T, qt) constant with pd, pv
+    proportional to p gives T proportional to p^(Rm/cpm), and the hydrostatic
+    equation d ln p/dz = -g/(Rm T) then gives T = Tg - g z/cpm and
+    p = Pg (T/Tg)^(cpm/Rm), with Rm = (1 - qt) Rd + qt Rv and
+    cpm = (1 - qt) cpd + qt cpv.  The reference state (ReferenceState.pyx:
+    39-135, odeint with hmax = 1) must match it on full and half levels to
+    the integrator's accuracy."""
+    from pycles_amd.ThermodynamicsSA import ThermodynamicsSA
+    nl = cases.namelist(4, 4, 80, 3, 50.0, 50.0, 25.0, moist=qtg > 0)
+    Gr = Grid(nl)
+    Th = ThermodynamicsSA(nl) if qtg > 0 else ThermodynamicsFactory(nl)
+    RS = ReferenceState(Gr)
+    RS.Pg, RS.Tg, RS.qtg = 1.0e5, 300.0, qtg
+    RS.initialize(Gr, Th)
+    gw = Gr.dims.gw
+    Rm = (1.0 - qtg) * P.Rd + qtg * P.Rv
+    cpm = (1.0 - qtg) * P.cpd + qtg * P.cpv
+
+    def adiabat(z):
+        T = RS.Tg - P.g * z / cpm
+        return RS.Pg * (T / RS.Tg) ** (cpm / Rm), T
+
+    # stay below the lifting condensation level in the moist column
+    top = 2000.0 if qtg > 0 else 2000.0
+    for z_arr, p_arr, a_arr in ((Gr.z_half, RS.p0_half, RS.alpha0_half), (Gr.z, RS.p0, RS.alpha0)):
+        sel = [k for k in range(gw, Gr.dims.nlg[2] - gw) if 0.0 <= z_arr[k] <= top]
+        assert len(sel) > 40
+        for k in sel:
+            p, T = adiabat(z_arr[k])
+            assert p_arr[k] == pytest.approx(p, rel=2e-8), (k, z_arr[k])
+            assert a_arr[k] == pytest.approx(Rm * T / p, rel=2e-8), (k, z_arr[k])
+
+
 def test_layout_round_trip():
     nl = cases.namelist(6, 5, 4, 3, 1.0, 1.0, 1.0, False)
     Gr = Grid(nl)
