@@ -151,6 +151,40 @@ int pycles_diffusion(const pycles_dims *dims, const pycles_ref *ref, const void 
                      const pycles_scalar_base *base, const void *visc, const void *diff, const void *strain_mag,
                      const void *temperature, int flags, int precision, void *stream);
 
+/* ---- BOMEX case physics ------------------------------------------------ */
+/* Replaces SurfaceBomex.update with the moist branch of SurfaceBase.update
+ * (Surface.pyx:297-373, 100-183; entropyflux_from_thetaflux_qtflux and
+ * compute_windspeed, surface.h:95-106, 157-185): theta flux 8e-3 K m/s, qt
+ * flux 5.2e-5 m/s, u* = 0.28 m/s, no gustiness; the fluxes of u, v, s, qt are
+ * added to tend[0..3] at the first interior level times
+ * alpha0_half[gw]/alpha0[gw-1]/dz.  qt, T, qv, ql at the first level;
+ * base: the FP32 profile of qt (base->base[1]).  flux2d (device FP64
+ * [8][nx*ny], may be NULL) receives the column fluxes u, v, s, qt and the
+ * diagnostics shf, lhf, buoyancy flux and Obukhov length, for which t_mean
+ * (the level means of T, device FP64 [nzg]) is read. */
+int pycles_surface_bomex(const pycles_dims *dims, const pycles_ref *ref, const void *u, const void *v,
+                         const void *qt, const void *T, const void *qv, const void *ql,
+                         const pycles_scalar_base *base, void *const tend[4], double u0, double v0,
+                         const double *t_mean, double *flux2d, int precision, void *stream);
+/* Replaces ForcingBomex.update (Forcing.pyx:147-203): the geostrophic
+ * pressure gradient with Coriolis (large_scale_p_gradient, :1982-2011) from
+ * the level means uv_mean = [2][nzg] of u and v, the large-scale entropy and
+ * moisture tendencies (s_tendency_c, entropies.h:17-21) and the subsidence of
+ * s, qt, u, v (apply_subsidence, Forcing.pyx:2013-2040); profiles = [5][nzg]
+ * ug, vg, dT/dt, dqt/dt, subsidence velocity as ForcingBomex.initialize
+ * builds them (Forcing.pyx:105-145).  tend = u, v, s, qt tendencies. */
+int pycles_forcing_bomex(const pycles_dims *dims, const pycles_ref *ref, const double *profiles,
+                         const double *uv_mean, const void *u, const void *v, const void *s, const void *qt,
+                         const void *T, const void *ql, const pycles_scalar_base *base, void *const tend[4],
+                         double coriolis, double u0, double v0, int precision, void *stream);
+/* Replaces Rayleigh.update (Damping.pyx:95-133) for nv (<= 8) fields:
+ * tend -= (phi - mean_k(phi)) gamma[v][k] over the interior, means = [nv][nzg]
+ * the level means of the stored fields (pycles_level_mean), gamma[v] the
+ * profile of field v (the reference uses the half-level profile for w and
+ * the full-level one for the others). */
+int pycles_rayleigh_damping(const pycles_dims *dims, int nv, const void *const *fields, void *const *tend,
+                            const double *const *gamma, const double *means, int precision, void *stream);
+
 /* ---- thermodynamics ---------------------------------------------------- */
 /* pv*(T) lookup table (reference Csrc/lookup.h:4-54). */
 typedef struct pycles_lookup {

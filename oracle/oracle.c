@@ -791,6 +791,210 @@ void orc_momentum_diffusion_update(const orc_dims *d, const orc_ref *rs, const d
 }
 
 /* ------------------------------------------------------------------------ */
+/* BOMEX case physics: surface fluxes, large-scale forcing, Rayleigh damping  */
+
+/* entropyflux_from_thetaflux_qtflux, surface.h:95-106 */
+double orc_entropyflux_from_thetaflux_qtflux(double thetaflux, double qtflux, double p0_b, double T_b, double qt_b,
+                                             double qv_b) {
+    const double exner_b = orc_exner(p0_b);
+    const double pd_b = pd_c(p0_b, qt_b, qv_b);
+    const double pv_b = pv_c(p0_b, qt_b, qv_b);
+    const double sd_b = sd_c(pd_b, T_b);
+    const double sv_b = sv_c(pv_b, T_b);
+    const double cp_b = cpm_c(qt_b);
+    return cp_b * thetaflux * exner_b / T_b + qtflux * (sv_b - sd_b);
+}
+
+/* SurfaceBomex.update (Surface.pyx:326-373) and the moist branch of
+ * SurfaceBase.update (Surface.pyx:150-183): the surface fluxes of s, qt, u
+ * and v, added to the tendencies at the first interior level, and the
+ * column diagnostics.  gustiness = 0 (SurfaceBomex does not set it).
+ * flux2d (may be NULL): [8][nlg0*nlg1] = u_flux, v_flux, s_flux, qt_flux,
+ * shf, lhf, b_flux, obukhov_length; t_mean_gw = horizontal mean of T at gw;
+ * L is the latent heat of the run (Microphysics.pxd:65-75). */
+void orc_surface_bomex(const orc_dims *d, const orc_ref *rs, const double *u, const double *v, const double *qt,
+                       const double *T, const double *qv, const double *ql, double *ut, double *vt, double *st,
+                       double *qtt, double u0, double v0, double t_mean_gw, int lh_constant, double *flux2d) {
+    long stt[3];
+    strides(d, stt);
+    const long gw = d->gw, n0 = d->nlg[0], n1 = d->nlg[1], n2d = n0 * n1;
+    const double theta_flux = 8.0e-3, ustar = 0.28, qt_flux = 5.2e-5, gustiness = 0.0;
+    double *uf = calloc(n2d, sizeof(double)), *vf = calloc(n2d, sizeof(double)), *sf = calloc(n2d, sizeof(double));
+    double *ws = calloc(n2d, sizeof(double));
+    for (long i = 0; i < n0; ++i)
+        for (long j = 0; j < n1; ++j) {
+            const long ijk = i * stt[0] + j * stt[1] + gw, ij = i * n1 + j;
+            sf[ij] = orc_entropyflux_from_thetaflux_qtflux(theta_flux, qt_flux, rs->p0_half[gw], T[ijk], qt[ijk], qv[ijk]);
+        }
+    /* compute_windspeed, surface.h:157-185 */
+    for (long i = 1; i < n0; ++i)
+        for (long j = 1; j < n1; ++j) {
+            const long ijk = i * stt[0] + j * stt[1] + gw, ij = i * n1 + j;
+            const double ui = orc_interp_2(u[ijk - stt[0]], u[ijk]) + u0;
+            const double vi = orc_interp_2(v[ijk - stt[1]], v[ijk]) + v0;
+            ws[ij] = fmax(sqrt(ui * ui + vi * vi), gustiness);
+        }
+    /* -ustar_**2 as the square of ustar */
+    for (long i = 1; i < n0 - 1; ++i)
+        for (long j = 1; j < n1 - 1; ++j) {
+            const long ijk = i * stt[0] + j * stt[1] + gw, ij = i * n1 + j;
+            uf[ij] = -(ustar * ustar) / orc_interp_2(ws[ij], ws[ij + n1]) * (u[ijk] + u0);
+            vf[ij] = -(ustar * ustar) / orc_interp_2(ws[ij], ws[ij + 1]) * (v[ijk] + v0);
+        }
+    const double dzi = 1.0 / d->dx[2];
+    const double tf = rs->alpha0_half[gw] / rs->alpha0[gw - 1] * dzi;
+    for (long i = gw; i < n0 - gw; ++i)
+        for (long j = gw; j < n1 - gw; ++j) {
+            const long ijk = i * stt[0] + j * stt[1] + gw, ij = i * n1 + j;
+            const double lv = lh_constant ? 2.501e6 : orc_latent_heat_variable(T[ijk]);
+            const double lhf = qt_flux * rs->rho0_half[gw] * lv;
+            const double pv = pv_c(rs->p0_half[gw], qt[ijk], qt[ijk] - ql[ijk]);
+            const double pd = pd_c(rs->p0_half[gw], qt[ijk], qt[ijk] - ql[ijk]);
+            const double sv = sv_c(pv, T[ijk]);
+            const double sd = sd_c(pd, T[ijk]);
+            const double shf = (sf[ij] * rs->rho0_half[gw] - lhf / lv * (sv - sd)) * T[ijk];
+            const double cp_ = cpm_c(qt[ijk]);
+            const double bf = C_G * rs->alpha0_half[gw] / cp_ / t_mean_gw *
+                              (shf + (C_EPS_VI - 1.0) * cp_ * t_mean_gw * lhf / lv);
+            const double lmo = -ustar * ustar * ustar / bf / 0.35; /* vkb, generate_parameters.py:44 */
+            ut[ijk] += uf[ij] * tf;
+            vt[ijk] += vf[ij] * tf;
+            st[ijk] += sf[ij] * tf;
+            qtt[ijk] += qt_flux * tf;
+            if (flux2d) {
+                flux2d[0 * n2d + ij] = uf[ij];
+                flux2d[1 * n2d + ij] = vf[ij];
+                flux2d[2 * n2d + ij] = sf[ij];
+                flux2d[3 * n2d + ij] = qt_flux;
+                flux2d[4 * n2d + ij] = shf;
+                flux2d[5 * n2d + ij] = lhf;
+                flux2d[6 * n2d + ij] = bf;
+                flux2d[7 * n2d + ij] = lmo;
+            }
+        }
+    free(uf);
+    free(vf);
+    free(sf);
+    free(ws);
+}
+
+/* ForcingBomex.initialize (Forcing.pyx:105-145): the profiles on the
+ * ghosted levels, prof = [5][nlg2] = ug, vg, dtdt, dqtdt, subsidence. */
+void orc_forcing_bomex_profiles(const orc_dims *d, const double *zl_half, const double *p0_half, double *prof) {
+    const long n2 = d->nlg[2];
+    double *ug = prof, *vg = prof + n2, *dtdt = prof + 2 * n2, *dqtdt = prof + 3 * n2, *sub = prof + 4 * n2;
+    for (long k = 0; k < n2; ++k) {
+        const double z = zl_half[k];
+        ug[k] = -10.0 + (1.8e-3) * z;
+        vg[k] = 0.0;
+        dtdt[k] = 0.0;
+        dqtdt[k] = 0.0;
+        sub[k] = 0.0;
+        if (z <= 1500.0) dtdt[k] = (-2.0 / (3600 * 24.0)) * orc_exner(p0_half[k]);
+        if (z > 1500.0)
+            dtdt[k] = (-2.0 / (3600 * 24.0) + (z - 1500.0) * (0.0 - -2.0 / (3600 * 24.0)) / (3000.0 - 1500.0)) *
+                      orc_exner(p0_half[k]);
+        if (z <= 300.0) dqtdt[k] = -1.2e-8;
+        if (z > 300.0 && z <= 500.0) dqtdt[k] = -1.2e-8 + (z - 300.0) * (0.0 - -1.2e-8) / (500.0 - 300.0);
+        if (z <= 1500.0) sub[k] = 0.0 + z * (-0.65 / 100.0 - 0.0) / (1500.0 - 0.0);
+        if (z > 1500.0 && z <= 2100.0) sub[k] = -0.65 / 100 + (z - 1500.0) * (0.0 - -0.65 / 100.0) / (2100.0 - 1500.0);
+    }
+}
+
+/* apply_subsidence, Forcing.pyx:2013-2040: the last interior level and the
+ * ghost levels above it repeat the tendency of the level below. */
+static void apply_subsidence(const orc_dims *d, const double *sub, const double *values, double *tend) {
+    long st[3];
+    strides(d, st);
+    const long gw = d->gw, kmax = d->nlg[2] - gw - 1;
+    const double dxi = d->dxi[2];
+#pragma omp parallel for schedule(static)
+    for (long i = gw; i < d->nlg[0] - gw; ++i)
+        for (long j = gw; j < d->nlg[1] - gw; ++j) {
+            double t = 0.0;
+            long k = gw;
+            for (; k < kmax; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                t = (values[ijk + 1] - values[ijk]) * dxi * sub[k];
+                tend[ijk] -= t;
+            }
+            for (; k < d->nlg[2]; ++k) tend[i * st[0] + j * st[1] + k] -= t;
+        }
+}
+
+/* ForcingBomex.update (Forcing.pyx:147-203): large_scale_p_gradient
+ * (:1982-2011) with the horizontal means of u and v, the large-scale
+ * entropy and moisture tendencies (s_tendency_c, entropies.h:17-21), then
+ * subsidence of s, qt, u, v. */
+void orc_forcing_bomex(const orc_dims *d, const orc_ref *rs, const double *prof, const double *u, const double *v,
+                       const double *s, const double *qt, const double *T, const double *ql, double *ut, double *vt,
+                       double *stend, double *qtt, double coriolis, double u0, double v0) {
+    long st[3];
+    strides(d, st);
+    const long gw = d->gw, n2 = d->nlg[2];
+    const double *ug = prof, *vg = prof + n2, *dtdt = prof + 2 * n2, *dqtdt = prof + 3 * n2, *sub = prof + 4 * n2;
+    double *umean = calloc(n2, sizeof(double)), *vmean = calloc(n2, sizeof(double));
+    orc_horizontal_mean(d, u, umean);
+    orc_horizontal_mean(d, v, vmean);
+#pragma omp parallel for schedule(static)
+    for (long i = gw; i < d->nlg[0] - gw; ++i)
+        for (long j = gw; j < d->nlg[1] - gw; ++j)
+            for (long k = gw; k < n2 - gw; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                const double u_at_v = umean[k] + u0, v_at_u = vmean[k] + v0;
+                ut[ijk] = ut[ijk] - coriolis * (vg[k] - v_at_u);
+                vt[ijk] = vt[ijk] + coriolis * (ug[k] - u_at_v);
+            }
+#pragma omp parallel for schedule(static)
+    for (long i = gw; i < d->nlg[0] - gw; ++i)
+        for (long j = gw; j < d->nlg[1] - gw; ++j)
+            for (long k = gw; k < n2 - gw; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                const double p0 = rs->p0_half[k], q = qt[ijk], qv = q - ql[ijk], t = T[ijk];
+                const double pv = pv_c(p0, q, qv), pd = pd_c(p0, q, qv);
+                stend[ijk] += cpm_c(q) * dtdt[k] / t + (sv_c(pv, t) - sd_c(pd, t)) * dqtdt[k];
+                qtt[ijk] += dqtdt[k];
+            }
+    apply_subsidence(d, sub, s, stend);
+    apply_subsidence(d, sub, qt, qtt);
+    apply_subsidence(d, sub, u, ut);
+    apply_subsidence(d, sub, v, vt);
+    free(umean);
+    free(vmean);
+}
+
+/* Rayleigh.initialize (Damping.pyx:71-93): gamma on half and full levels. */
+void orc_rayleigh_profiles(const orc_dims *d, const double *zl, const double *zl_half, double gamma_r, double z_d,
+                           double *gamma_z, double *gamma_zhalf) {
+    const double z_top = d->dx[2] * d->n[2];
+    for (long k = 0; k < d->nlg[2]; ++k) {
+        gamma_zhalf[k] = 0.0;
+        gamma_z[k] = 0.0;
+        if (zl_half[k] >= z_top - z_d)
+            gamma_zhalf[k] = gamma_r * pow(sin((C_PI / 2.0) * (1.0 - (z_top - zl_half[k]) / z_d)), 2.0);
+        if (zl[k] >= z_top - z_d) gamma_z[k] = gamma_r * pow(sin((C_PI / 2.0) * (1.0 - (z_top - zl[k]) / z_d)), 2.0);
+    }
+}
+
+/* Rayleigh.update (Damping.pyx:95-133) of one variable: w uses the
+ * half-level profile, the others the full-level one, as the reference. */
+void orc_rayleigh_update(const orc_dims *d, const double *gamma, const double *values, double *tend) {
+    long st[3];
+    strides(d, st);
+    const long gw = d->gw, n2 = d->nlg[2];
+    double *mean = calloc(n2, sizeof(double));
+    orc_horizontal_mean(d, values, mean);
+#pragma omp parallel for schedule(static)
+    for (long i = gw; i < d->nlg[0] - gw; ++i)
+        for (long j = gw; j < d->nlg[1] - gw; ++j)
+            for (long k = gw; k < n2 - gw; ++k) {
+                const long ijk = i * st[0] + j * st[1] + k;
+                tend[ijk] -= (values[ijk] - mean[k]) * gamma[k];
+            }
+    free(mean);
+}
+
+/* ------------------------------------------------------------------------ */
 /* State, boundaries, time stepping                                          */
 
 /* One variable: periodic x then periodic y (build_buffer/buffer_to_values

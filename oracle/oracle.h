@@ -136,6 +136,19 @@ void orc_scalar_diffusion_update(const orc_dims *d, const orc_ref *rs, const dou
 void orc_momentum_diffusion_update(const orc_dims *d, const orc_ref *rs, const double *strain,
                                    const double *visc, double *const *tend, double *flux, const double *smag,
                                    const double *T, double *s_tend);
+/* ---- BOMEX case physics ---- */
+double orc_entropyflux_from_thetaflux_qtflux(double thetaflux, double qtflux, double p0_b, double T_b, double qt_b,
+                                             double qv_b);
+void orc_surface_bomex(const orc_dims *d, const orc_ref *rs, const double *u, const double *v, const double *qt,
+                       const double *T, const double *qv, const double *ql, double *ut, double *vt, double *st,
+                       double *qtt, double u0, double v0, double t_mean_gw, int lh_constant, double *flux2d);
+void orc_forcing_bomex_profiles(const orc_dims *d, const double *zl_half, const double *p0_half, double *prof);
+void orc_forcing_bomex(const orc_dims *d, const orc_ref *rs, const double *prof, const double *u, const double *v,
+                       const double *s, const double *qt, const double *T, const double *ql, double *ut, double *vt,
+                       double *stend, double *qtt, double coriolis, double u0, double v0);
+void orc_rayleigh_profiles(const orc_dims *d, const double *zl, const double *zl_half, double gamma_r, double z_d,
+                           double *gamma_z, double *gamma_zhalf);
+void orc_rayleigh_update(const orc_dims *d, const double *gamma, const double *values, double *tend);
 void orc_update_bcs(const orc_dims *d, double *values, double bc_factor);
 /* TimeStepping.update_third (TimeStepping.pyx:131-151) over n = npg*nv points. */
 void orc_rk3_update(long n, int rk_step, double dt, double *values, double *tend,

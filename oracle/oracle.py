@@ -87,6 +87,13 @@ def lib():
         L.orc_tdma_solve.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp]
         L.orc_fft.argtypes = [C.c_long, _dp, _dp, C.c_int]
         L.orc_update_bcs.argtypes = [C.POINTER(OrcDims), _dp, d]
+        L.orc_surface_bomex.argtypes = [C.POINTER(OrcDims), C.POINTER(OrcRef)] + [_dp] * 10 + [d, d, d, C.c_int, _dp]
+        L.orc_forcing_bomex_profiles.argtypes = [C.POINTER(OrcDims), _dp, _dp, _dp]
+        L.orc_forcing_bomex.argtypes = [C.POINTER(OrcDims), C.POINTER(OrcRef)] + [_dp] * 11 + [d, d, d]
+        L.orc_rayleigh_profiles.argtypes = [C.POINTER(OrcDims), _dp, _dp, d, d, _dp, _dp]
+        L.orc_rayleigh_update.argtypes = [C.POINTER(OrcDims), _dp, _dp, _dp]
+        L.orc_entropyflux_from_thetaflux_qtflux.restype = d
+        L.orc_entropyflux_from_thetaflux_qtflux.argtypes = [d] * 6
         L.orc_kinematics_update.argtypes = [C.POINTER(OrcDims), C.POINTER(_dp), _dp, _dp, _dp]
         L.orc_smagorinsky_update.argtypes = [C.POINTER(OrcDims), _dp, _dp, _dp, _dp, d, d]
         L.orc_scalar_diffusion_update.argtypes = [C.POINTER(OrcDims), C.POINTER(OrcRef), _dp, C.c_int,
@@ -183,7 +190,7 @@ class OracleSimulation:
 
     def __init__(self, dims: OrcDims, ref: Ref, moist: bool, lookup: Lookup | None,
                  scalar_order=5, momentum_order=5, dt=1.0, dt_max=None, cfl_limit=1e6,
-                 t_max=1e30, max_iter=0, sgs=None):
+                 t_max=1e30, max_iter=0, sgs=None, bomex=None):
         self.d = dims
         self.ref = ref
         self.moist = moist
@@ -208,6 +215,20 @@ class OracleSimulation:
         self.max_iter = max_iter
         self.cfl_max = 0.0
         self.sa_stats = OrcSAStats()
+        # BOMEX surface, forcing and Rayleigh damping: {"zl", "zl_half", "u0",
+        # "v0", "gamma_r", "z_d"} (the grid's full and half level heights)
+        self.bomex = bomex
+        if bomex is not None:
+            n2 = dims.nlg[2]
+            self.forcing_prof = np.zeros(5 * n2)
+            lib().orc_forcing_bomex_profiles(C.byref(dims), ptr(np.ascontiguousarray(bomex["zl_half"])),
+                                             ptr(ref.arrays[1]), ptr(self.forcing_prof))
+            self.gamma_z = np.zeros(n2)
+            self.gamma_zhalf = np.zeros(n2)
+            lib().orc_rayleigh_profiles(C.byref(dims), ptr(np.ascontiguousarray(bomex["zl"])),
+                                        ptr(np.ascontiguousarray(bomex["zl_half"])), bomex["gamma_r"],
+                                        bomex["z_d"], ptr(self.gamma_z), ptr(self.gamma_zhalf))
+            self.surface_flux2d = np.zeros(8 * dims.nlg[0] * dims.nlg[1])
         # Smagorinsky SGS with scalar and momentum diffusion: {"cs", "prt"}
         self.sgs = sgs
         if sgs is not None:
@@ -284,6 +305,30 @@ class OracleSimulation:
                                             ptr(self.dv["viscosity"]), tend, ptr(self.flux), ptr(self.smag),
                                             ptr(self.dv["temperature"]), ptr(self.pt("s")))
 
+    def surface_update(self):
+        """SurfaceBomex.update (Surface.pyx:326-373)."""
+        tmean = np.zeros(self.shape[2])
+        lib().orc_horizontal_mean(C.byref(self.d), ptr(self.dv["temperature"]), ptr(tmean))
+        dv = self.dv
+        lib().orc_surface_bomex(C.byref(self.d), C.byref(self.ref.c), ptr(self.pv("u")), ptr(self.pv("v")),
+                                ptr(self.pv("qt")), ptr(dv["temperature"]), ptr(dv["qv"]), ptr(dv["ql"]),
+                                ptr(self.pt("u")), ptr(self.pt("v")), ptr(self.pt("s")), ptr(self.pt("qt")),
+                                self.bomex["u0"], self.bomex["v0"], tmean[self.d.gw], 0, ptr(self.surface_flux2d))
+
+    def damping_update(self):
+        """Rayleigh.update (Damping.pyx:95-133): w with the half-level profile."""
+        for n in self.pv_names:
+            g = self.gamma_zhalf if n == "w" else self.gamma_z
+            lib().orc_rayleigh_update(C.byref(self.d), ptr(g), ptr(self.pv(n)), ptr(self.pt(n)))
+
+    def forcing_update(self):
+        """ForcingBomex.update (Forcing.pyx:147-203)."""
+        dv = self.dv
+        lib().orc_forcing_bomex(C.byref(self.d), C.byref(self.ref.c), ptr(self.forcing_prof), ptr(self.pv("u")),
+                                ptr(self.pv("v")), ptr(self.pv("s")), ptr(self.pv("qt")), ptr(dv["temperature"]),
+                                ptr(dv["ql"]), ptr(self.pt("u")), ptr(self.pt("v")), ptr(self.pt("s")),
+                                ptr(self.pt("qt")), 0.376e-4, self.bomex["u0"], self.bomex["v0"])
+
     def rk_update(self, rk_step):
         lib().orc_rk3_update(self.values.size, rk_step, self.dt, ptr(self.values), ptr(self.tend),
                              ptr(self.copy))
@@ -313,10 +358,17 @@ class OracleSimulation:
         self.thermo_update()
         self.scalar_advection_update()
         self.momentum_advection_update()
+        if self.bomex is not None:
+            self.surface_update()
         if self.sgs is not None:
             self.sgs_update()
+        if self.bomex is not None:
+            self.damping_update()
+        if self.sgs is not None:
             self.scalar_diffusion_update()
             self.momentum_diffusion_update()
+        if self.bomex is not None:
+            self.forcing_update()
         self.rk_update(rk_step)
         self.update_all_bcs()
         self.pressure_update()

@@ -4,13 +4,15 @@
 (``Simulation3d.pyx:40-138``) restricted to the hot path; ``run()`` follows
 ``:140-189``: one boundary fill and one thermodynamics update before the loop
 (the latter adds buoyancy to the w tendency, which the first stage keeps), then
-for each RK stage Ke.update, Th.update, SA.update, MA.update, SGS.update,
-SD.update and MD.update (with an sgs section in the namelist), TS.update (with
-the z boundary fill), Pr.update and, on the last stage, TS.adjust_timestep and
-the restart and statistics output.  The operators outside this path
-(surface, forcing, radiation, damping, tracers, microphysics processes) are
-not part of this build and contribute nothing, as with a namelist that
-disables them.
+for each RK stage Ke.update, Th.update, SA.update, MA.update, Sur.update,
+SGS.update, Damping.update, SD.update and MD.update, Fo.update (the SGS and
+diffusion operators with an sgs section in the namelist; surface and forcing
+for the Bomex case; damping with a damping section), TS.update (with the z
+boundary fill), Pr.update and, on the last stage, TS.adjust_timestep and the
+restart and statistics output.  The operators outside this path (radiation,
+tracers, microphysics processes, other cases' surface and forcing) are not
+part of this build and contribute nothing, as with a namelist that disables
+them.
 """
 from __future__ import annotations
 
@@ -63,6 +65,14 @@ class Simulation3d:
             self.SGS = SGS(namelist)
             self.SD = ScalarDiffusion(namelist, self.LH, self.DV, self.Pa)
             self.MD = MomentumDiffusion(self.DV, self.Pa)
+        # case physics: surface, large-scale forcing and damping by the
+        # namelist's casename and damping section (Simulation3d.pyx:54-62)
+        from .CasePhysics import Damping, Dummy, Forcing, ForcingNone, SurfaceFactory, SurfaceNone
+        self.Sur = SurfaceFactory(namelist, self.LH, self.Pa)
+        self.Fo = Forcing(namelist, self.LH, self.Pa)
+        self.Damping = Damping(namelist, self.Pa)
+        self.case_physics = not (isinstance(self.Sur, SurfaceNone) and isinstance(self.Fo.scheme, ForcingNone)
+                                 and isinstance(self.Damping.scheme, Dummy))
         self.TS = TimeStepping()
         self.StatsIO = NetCDFIO_Stats().initialize(namelist, None, None)
         self.Restart = Restart(namelist, self.Pa)
@@ -90,6 +100,8 @@ class Simulation3d:
             self.PV.init_from_restart(self.Gr, self.Restart)
             if hasattr(self.Th, "init_from_restart"):
                 self.Th.init_from_restart(self.Restart)
+            if hasattr(self.Sur, "init_from_restart") and "surf" in self.Restart.restart_data:
+                self.Sur.init_from_restart(self.Restart)
             self.Restart.last_restart_time = self.Restart.restart_data["last_restart_time"]
             self.Restart.free_memory()
             fields = {}
@@ -101,6 +113,9 @@ class Simulation3d:
                     self.PV.set_base_from_device_layout(name, arr)
                 self.PV.set_device_layout(name, arr)
         self.initial_fields = fields
+        self.Sur.initialize(self.Gr, self.Ref, self.StatsIO, self.Pa)
+        self.Fo.initialize(self.Gr, self.Ref, self.Th, self.StatsIO, self.Pa)
+        self.Damping.initialize(self.Gr, self.Ref)
         self.Pr.initialize(namelist, self.Gr, self.Ref, self.DV, self.Pa)
         self.DV.initialize(self.Gr, self.StatsIO, self.Pa, precision=precision)
         self.walltimes = []
@@ -147,7 +162,8 @@ class Simulation3d:
         self._wt_pending = False
         # with SGS and diffusion the RK update runs after them (their
         # tendencies add to the advection's first, as in the reference)
-        fused_rk = self.fuse_rk and self.fuse_advection and fast and self.SGS is None
+        fused_rk = (self.fuse_rk and self.fuse_advection and fast and self.SGS is None
+                    and not self.case_physics)
         if fused_rk:
             if self.timers is not None:
                 from ._device import _capi
@@ -177,12 +193,23 @@ class Simulation3d:
             self._mark("advection")
             self.SA.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa, tend_zero=fast)
             self.MA.update(self.Gr, self.Ref, self.PV, self.Pa, tend_zero=fast)
+        # Simulation3d.pyx:165-171: Sur, SGS, Damping, SD, MD, Fo
+        if self.case_physics:
+            self._mark("surface")
+            self.Sur.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa, self.TS)
+        if self.SGS is not None:
+            self._mark("sgs")
+            self.SGS.update(self.Gr, self.DV, self.PV, self.Ke, self.Sur, self.Pa)
+        if self.case_physics:
+            self._mark("damping")
+            self.Damping.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa)
         if self.SGS is not None:
             from .SGS import diffuse
-            self._mark("sgs")
-            self.SGS.update(self.Gr, self.DV, self.PV, self.Ke, None, self.Pa)
             self._mark("diffusion")
             diffuse(self.SD, self.MD, self.Gr, self.Ref, self.PV, self.DV, self.Ke)
+        if self.case_physics:
+            self._mark("forcing")
+            self.Fo.update(self.Gr, self.Ref, self.PV, self.DV, self.Pa)
         if not fused_rk:
             self._mark("rk3")
             self.TS.update(self.Gr, self.PV, self.Pa, zero_tendencies=not fast)
@@ -226,6 +253,7 @@ class Simulation3d:
         self.DV.stats_io(self.Gr, NS, self.Pa)
         if hasattr(self.Th, "stats_io"):
             self.Th.stats_io(self.Gr, self.Ref, self.PV, self.DV, NS, self.Pa)
+        self.Sur.stats_io(self.Gr, NS, self.Pa)
         return NS.close_files(self.Pa)
 
     def write_restart(self):
@@ -236,6 +264,8 @@ class Simulation3d:
         self.Ref.restart(self.Gr, Re)
         self.PV.restart(self.Gr, Re)
         self.TS.restart(Re)
+        if hasattr(self.Sur, "restart"):
+            self.Sur.restart(Re)
         if hasattr(self.Th, "restart"):
             self.Th.restart(Re)
         return Re.write(self.Pa)

@@ -86,6 +86,16 @@ cdef extern from "pycles_amd.h":
     int pycles_cfl_max(const pycles_dims *dims, const void *u, const void *v, const void *w,
                        double dt, double *out, int precision, void *stream)
     int pycles_cfl_read(const double *dev_cfl, double *host_cfl, void *stream)
+    int pycles_surface_bomex(const pycles_dims *dims, const pycles_ref *ref, const void *u, const void *v,
+                             const void *qt, const void *T, const void *qv, const void *ql,
+                             const pycles_scalar_base *base, void *const *tend, double u0, double v0,
+                             const double *t_mean, double *flux2d, int precision, void *stream)
+    int pycles_forcing_bomex(const pycles_dims *dims, const pycles_ref *ref, const double *profiles,
+                             const double *uv_mean, const void *u, const void *v, const void *s, const void *qt,
+                             const void *T, const void *ql, const pycles_scalar_base *base, void *const *tend,
+                             double coriolis, double u0, double v0, int precision, void *stream)
+    int pycles_rayleigh_damping(const pycles_dims *dims, int nv, const void *const *fields, void *const *tend,
+                                const double *const *gamma, const double *means, int precision, void *stream)
     int pycles_sgs_smagorinsky(const pycles_dims *dims, const void *const *vel, const void *bvf, double cs,
                                double prt, void *strain_mag, void *visc, void *diff, int precision, void *stream)
     int pycles_diffusion(const pycles_dims *dims, const pycles_ref *ref, const void *const *vel,
@@ -404,6 +414,40 @@ def diffusion(Dims dims, Ref ref, vel, tend_vel, scal, tend_scal, ScalarBase bas
     _check(pycles_diffusion(&dims.d, &ref.r, v3, t3 if tend_vel is not None else NULL, ns, sc, ts,
                             &base.b if base is not None else NULL, _p(visc), _p(diff), _p(strain_mag),
                             _p(temperature), flags, precision, _p(stream)))
+
+
+def surface_bomex(Dims dims, Ref ref, u, v, qt, T, qv, ql, ScalarBase base, tend, double u0, double v0, t_mean,
+                  flux2d, int precision, stream):
+    cdef void *t4[4]
+    for i in range(4):
+        t4[i] = _p(tend[i])
+    _check(pycles_surface_bomex(&dims.d, &ref.r, _p(u), _p(v), _p(qt), _p(T), _p(qv), _p(ql),
+                                &base.b if base is not None else NULL, t4, u0, v0, <const double *> _p(t_mean),
+                                <double *> _p(flux2d), precision, _p(stream)))
+
+
+def forcing_bomex(Dims dims, Ref ref, profiles, uv_mean, u, v, s, qt, T, ql, ScalarBase base, tend,
+                  double coriolis, double u0, double v0, int precision, stream):
+    cdef void *t4[4]
+    for i in range(4):
+        t4[i] = _p(tend[i])
+    _check(pycles_forcing_bomex(&dims.d, &ref.r, <const double *> _p(profiles), <const double *> _p(uv_mean),
+                                _p(u), _p(v), _p(s), _p(qt), _p(T), _p(ql), &base.b if base is not None else NULL,
+                                t4, coriolis, u0, v0, precision, _p(stream)))
+
+
+def rayleigh_damping(Dims dims, fields, tend, gamma, means, int precision, stream):
+    cdef int nv = len(fields)
+    cdef const void *f[8]
+    cdef void *t[8]
+    cdef const double *gm[8]
+    if nv > 8:
+        raise ValueError("at most 8 fields")
+    for i in range(nv):
+        f[i] = _p(fields[i])
+        t[i] = _p(tend[i])
+        gm[i] = <const double *> _p(gamma[i])
+    _check(pycles_rayleigh_damping(&dims.d, nv, f, t, gm, <const double *> _p(means), precision, _p(stream)))
 
 
 def launch_chunks(int kernel, Dims dims):

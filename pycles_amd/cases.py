@@ -297,13 +297,94 @@ def init_moist(Gr, RS, Th, seed=2, sigma=0.3, cloud=None):
     return fields
 
 
+def bomex_profiles(z):
+    """InitBomex's liquid-water potential temperature, total water (kg/kg)
+    and u profiles (Initialization.pyx:296-329)."""
+    z = np.asarray(z, dtype=np.float64)
+    thetal = np.empty_like(z)
+    qt = np.empty_like(z)
+    u = np.empty_like(z)
+    for k, zk in enumerate(z):
+        if zk <= 520.0:
+            thetal[k] = 298.7
+        if 520.0 < zk <= 1480.0:
+            thetal[k] = 298.7 + (zk - 520) * (302.4 - 298.7) / (1480.0 - 520.0)
+        if 1480.0 < zk <= 2000:
+            thetal[k] = 302.4 + (zk - 1480.0) * (308.2 - 302.4) / (2000.0 - 1480.0)
+        if zk > 2000.0:
+            thetal[k] = 308.2 + (zk - 2000.0) * (311.85 - 308.2) / (3000.0 - 2000.0)
+        if zk <= 520:
+            qt[k] = 17.0 + zk * (16.3 - 17.0) / 520.0
+        if 520.0 < zk <= 1480.0:
+            qt[k] = 16.3 + (zk - 520.0) * (10.7 - 16.3) / (1480.0 - 520.0)
+        if 1480.0 < zk <= 2000.0:
+            qt[k] = 10.7 + (zk - 1480.0) * (4.2 - 10.7) / (2000.0 - 1480.0)
+        if zk > 2000.0:
+            qt[k] = 4.2 + (zk - 2000.0) * (3.0 - 4.2) / (3000.0 - 2000.0)
+        qt[k] /= 1000.0
+        if zk <= 700.0:
+            u[k] = -8.75
+        if zk > 700.0:
+            u[k] = -8.75 + (zk - 700.0) * (-4.61 - -8.75) / (3000.0 - 700.0)
+    return thetal, qt, u
+
+
+def init_bomex(Gr, RS, Th, seed=6):
+    """InitBomex (Initialization.pyx:260-368): reference state Pg = 1.015e5,
+    Tg = 300.4, qtg = 0.02245; the Galilean translation u0 = (max u + min u)/2;
+    u - u0, v = w = 0; theta_l + U(-0.05, 0.05) K and qt + U(-0.0125, 0.0125)
+    g/kg up to 1600 m; s = entropy(p0_half, theta_l exner, qt).  The noise
+    comes from the counter-based generator instead of numpy's global one."""
+    RS.Pg = 1.015e5
+    RS.Tg = 300.4
+    RS.qtg = 0.02245
+    RS.initialize(Gr, Th)
+    d = Gr.dims
+    shape = (d.nzg, d.n[1], d.n[0])
+    zh = Gr.zl_half[:, None, None]
+    p0h = RS.p0_half[:, None, None]
+    thetal, qt, u = bomex_profiles(Gr.zl_half)
+    RS.v0 = 0.0
+    RS.u0 = 0.5 * (np.amax(u) + np.amin(u))
+    pert = zh <= 1600.0
+    theta_pert = (uniform(seed, 30, shape) - 0.5) * 0.1
+    qt_pert = (uniform(seed, 31, shape) - 0.5) * 0.025 / 1000.0
+    temp = np.where(pert, (thetal[:, None, None] + theta_pert), thetal[:, None, None]) * H.exner(p0h)
+    q = np.where(pert, qt[:, None, None] + qt_pert, qt[:, None, None] + 0.0 * qt_pert)
+    fields = {"u": np.broadcast_to((u - RS.u0)[:, None, None], shape).copy(),
+              "v": np.full(shape, 0.0 - RS.v0), "w": np.zeros(shape),
+              "s": H.entropy_sa(p0h, temp, q, 0.0, 0.0), "qt": q}
+    for n, bc in (("u", 1.0), ("v", 1.0), ("w", -1.0), ("s", 1.0), ("qt", 1.0)):
+        fill_z_ghosts(fields[n], d.gw, bc)
+    return fields
+
+
+def bomex_namelist(nx=64, ny=64, nz=75, gw=3, dx=100.0, dz=40.0, dt=10.0, precision="f64", seed=6,
+                   cfl_limit=0.7, t_max=21600.0):
+    """The reference's Bomex namelist (generate_namelist.py:311-393) on the
+    device path: None_SA thermodynamics, Smagorinsky SGS, Rayleigh damping
+    (gamma_r = 0.2 /s over the top 600 m), WENO5 transport, CFL-limited
+    dt up to dt_max."""
+    nl = namelist(nx, ny, nz, gw, dx, dx, dz, True, order=(5, 5), dt=dt, t_max=t_max, precision=precision,
+                  cfl_limit=cfl_limit, seed=seed)
+    nl["time_stepping"]["dt_max"] = dt
+    nl["meta"] = {"casename": "Bomex", "simname": "Bomex", "uuid": "bomex-0000-00000"}
+    nl["sgs"] = {"scheme": "Smagorinsky"}
+    nl["diffusion"] = {"qt_entropy_source": False}
+    nl["damping"] = {"scheme": "Rayleigh", "Rayleigh": {"gamma_r": 0.2, "z_d": 600.0}}
+    nl["initialization"]["case"] = "bomex"
+    return nl
+
+
 def initialize(namelist, Gr, PV, RS, Th):
     """Build the initial state and upload it (the InitializationFactory step)."""
     ini = namelist.get("initialization", {})
     moist = namelist["microphysics"]["scheme"] == "None_SA"
     seed = int(ini.get("seed", 1))
     sigma = float(ini.get("sigma", 0.5))
-    if moist:
+    if ini.get("case") == "bomex":
+        fields = init_bomex(Gr, RS, Th, seed=seed)
+    elif moist:
         fields = init_moist(Gr, RS, Th, seed=seed, sigma=sigma, cloud=ini.get("cloud"))
     else:
         fields = init_dry(Gr, RS, Th, seed=seed, sigma=sigma)

@@ -20,13 +20,13 @@ def host_case(nx, ny, nz, gw=3, moist=True, dx=50.0, dz=20.0, seed=2, sigma=0.3,
     return nl, Gr, RS, Th, fields
 
 
-def oracle_sim(Gr, RS, Th, fields, moist, order=(5, 5), dt=1.0, max_iter=0, sgs=None):
+def oracle_sim(Gr, RS, Th, fields, moist, order=(5, 5), dt=1.0, max_iter=0, sgs=None, bomex=None, **kw):
     d = Gr.dims
     dims = O.make_dims(d.n[0], d.n[1], d.n[2], d.gw, *d.dx)
     ref = O.Ref(RS.p0, RS.p0_half, RS.alpha0, RS.alpha0_half, RS.rho0, RS.rho0_half)
     lt = O.Lookup(Th.CC.T, Th.CC.pv) if moist else None
     sim = O.OracleSimulation(dims, ref, moist, lt, scalar_order=order[0], momentum_order=order[1],
-                             dt=dt, max_iter=max_iter, sgs=sgs)
+                             dt=dt, max_iter=max_iter, sgs=sgs, bomex=bomex, **kw)
     for n in sim.pv_names:
         sim.pv(n)[...] = Gr.from_device_layout(fields[n])
     return sim
