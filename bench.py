@@ -272,6 +272,65 @@ def measured_traffic():
     return {}
 
 
+def cpu_reference_c1(repeats=5):
+    """The dry 64^3 configuration's SSP-RK3 step on one CPU core with the
+    reference's own compiled kernels where they build (momentum advection and
+    the boundary fill, oracle/_ref) and the FP64 oracle for the rest (scalar
+    advection, dry thermodynamics, RK update, pressure projection): the
+    reference's CPU path as one MPI rank would run it.  Median of `repeats`."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    import ctypes as C
+    from oracle import oracle as O
+    from oracle import reflib
+    from tests.helpers import oracle_sim
+    c, nl, Gr, RS, Th, fields = build_state("dry64")
+    d = Gr.dims
+    rd = reflib.make_dims(d.n[0], d.n[1], d.n[2], d.gw, *d.dx)
+    R = reflib.lib()
+    times = []
+    for _ in range(repeats):
+        sim = oracle_sim(Gr, RS, Th, fields, False, dt=c["dt"])
+        r = sim.ref.arrays  # p0, p0_half, alpha0, alpha0_half, rho0, rho0_half
+
+        def momentum():
+            for a in range(3):
+                for b in range(3):
+                    R.compute_advective_tendencies_m(C.byref(rd), reflib.ptr(r[4]), reflib.ptr(r[5]),
+                                                     reflib.ptr(r[2]), reflib.ptr(r[3]),
+                                                     reflib.ptr(sim.pv("uvw"[a])), reflib.ptr(sim.pv("uvw"[b])),
+                                                     reflib.ptr(sim.pt("uvw"[a])), a, b, c["order"][1])
+
+        def bcs():
+            for n in sim.pv_names:
+                reflib.update_all_bcs(rd, sim.pv(n), O.BC[n])
+
+        sim.momentum_advection_update = momentum
+        sim.update_all_bcs = bcs
+        sim.begin_run()
+        t0 = time.perf_counter()
+        sim.step()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    n = c["nx"] * c["ny"] * c["nz"]
+    return {"metric": "Mcell-updates/s per RK3 step (reference CPU path)", "value": round(n / t / 1e6, 4),
+            "unit": "Mcell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"dry 64^3, WENO5, one SSP-RK3 step, dt = 1 s, median of {repeats}: {t:.3f} s; "
+                      "momentum advection and boundary fill from the reference's compiled C (oracle/_ref), "
+                      "scalar advection, dry thermodynamics, RK update and pressure from the oracle (C, 1 thread)",
+            "seconds_per_step": t, "cpu": _cpu_name()}
+
+
+def _cpu_name():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,7 +348,12 @@ def main():
     ap.add_argument("--load-guess", default=None, help="use this trained network (npz) instead of training")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for HBM counter scaling)")
+    ap.add_argument("--cpu-reference-c1", action="store_true",
+                    help="only time the reference's CPU path on the dry 64^3 configuration, one core (no GPU)")
     args = ap.parse_args()
+    if args.cpu_reference_c1:
+        print(json.dumps(cpu_reference_c1()), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
               "decomposition is a design only).", file=sys.stderr)
