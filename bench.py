@@ -247,6 +247,42 @@ def per_kernel(phases, steps):
     return out
 
 
+def hbm_ceiling(nbytes=1 << 30, reps=5):
+    """The HBM rate the device reaches with the library's streaming kernels
+    (16-byte loads and stores, grid-stride, plain and non-temporal) on
+    `nbytes` buffers: copy, read-only and write-only, over a range of grid
+    sizes; the best average launch time of `reps` per variant, timed with
+    HIP events on the launching stream."""
+    import torch
+    from pycles_amd import _capi
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    st = torch.cuda.current_stream()
+    out = {"bytes": nbytes, "variants": {}}
+    best = {}
+    for mode, name, moved in ((0, "copy", 2), (1, "read", 1), (2, "write", 1)):
+        for nt in (0, 4):
+            for blocks in (1024, 4096, 16384, 65536):
+                _capi.stream_kernel(mode + nt, a.data_ptr(), b.data_ptr(), nbytes, blocks, st.cuda_stream)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                for _ in range(reps):
+                    _capi.stream_kernel(mode + nt, a.data_ptr(), b.data_ptr(), nbytes, blocks, st.cuda_stream)
+                e1.record(st)
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                gbs = moved * nbytes / (ms * 1e-3) / 1e9
+                key = f"{name}{'_nt' if nt else ''}_g{blocks}"
+                out["variants"][key] = round(gbs, 1)
+                if gbs > best.get(name, (0.0, ""))[0]:
+                    best[name] = (gbs, key)
+    out["best_GBs"] = {k: round(v[0], 1) for k, v in best.items()}
+    out["best_variant"] = {k: v[1] for k, v in best.items()}
+    del a, b
+    return out
+
+
 def cpu_baseline(c, Gr, RS, Th, fields):
     """FP64 CPU oracle: one SSP-RK3 step of the same workload on the host cores."""
     from tests.helpers import oracle_sim
@@ -343,6 +379,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--guess-pairs-log2", type=int, default=22)
     ap.add_argument("--guess-epochs", type=int, default=40)
+    ap.add_argument("--newton-steps", type=int, default=None,
+                    help="Newton corrections after the learned guess (default: the thermodynamics' setting)")
     ap.add_argument("--no-floor", action="store_true", help="skip the FP64 noise-floor runs of the error study")
     ap.add_argument("--save-guess", default=None, help="write the trained network (npz) here")
     ap.add_argument("--load-guess", default=None, help="use this trained network (npz) instead of training")
@@ -366,13 +404,18 @@ def main():
     mcell = lambda ms: ncell / (ms * 1e-3) / 1e6  # noqa: E731
 
     if args.copy_calibration:
+        from pycles_amd import _capi
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        _capi.stream_kernel(0, a.data_ptr(), b.data_ptr(), 1 << 30, 16384, torch.cuda.current_stream().cuda_stream)
         torch.cuda.synchronize()
-        log("[bench] copy calibration: 1 GiB read + 1 GiB written")
+        log("[bench] copy calibration: 1 GiB read + 1 GiB written (pycles_stream copy, 16384 blocks)")
         del a, b
+    ceiling = hbm_ceiling()
+    log(f"[bench] HBM ceiling of the streaming kernels: {ceiling['best_GBs']} GB/s")
+    if args.newton_steps is not None:
+        Th.newton_steps = args.newton_steps
     if args.load_guess:
         guess, ginfo = load_guess(Th, args.load_guess)
         log(f"[bench] loaded the saturation guess from {args.load_guess}")
@@ -423,6 +466,7 @@ def main():
     tr = traffic.get(dom)
     roof = {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kd["hbm_frac"],
+            "frac_of_copy_ceiling": round(kd["achieved_GBs"] / ceiling["best_GBs"]["copy"], 4),
             "traffic": tr["bytes_per_launch"] if tr else None,
             "algorithmic_bytes_per_launch": kd["bytes_per_cell"] * ncell, "ms_per_launch": kd["ms_per_launch"]}
     # the whole step against its algorithmic bytes: 186 field passes per
@@ -444,7 +488,8 @@ def main():
                                   f"{c['order'][1]}, FFT+tridiagonal pressure, dt={c['dt']} s, "
                                   "FP32 + learned saturation guess", "cells": ncell},
            "phases_ms_per_step": {k: round(v / args.steps, 4) for k, v in phL.items()},
-           "roofline": roof, "step_roofline": step_roof, "kernels": kernels, "learned_guess": ginfo}
+           "roofline": roof, "step_roofline": step_roof, "hbm_ceiling": ceiling, "kernels": kernels,
+           "learned_guess": ginfo}
     if out64:
         out["fp64"] = out64
         out["speedup_vs_fp64"] = round(out64["ms_per_step"] / msL, 3)

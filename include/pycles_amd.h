@@ -343,6 +343,17 @@ int pycles_cloud_stats(const pycles_dims *dims, const double *rho0_half, const v
  * (all nzg levels, as 1).  No reference counterpart. */
 int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len);
 
+/* ---- HBM calibration (no reference counterpart) ------------------------ */
+/* Streaming kernels of known traffic, for the practical HBM ceiling and for
+ * scaling the FETCH_SIZE / WRITE_SIZE counters.  mode 0 = copy src to dst
+ * (bytes read and bytes written), 1 = read src (dst receives one 4-byte word
+ * per block, so it holds >= 4*blocks bytes), 2 = fill dst (bytes written);
+ * mode + 4 uses non-temporal loads and stores.  bytes: a positive multiple
+ * of 16; src, dst: 16-byte aligned device pointers; blocks: grid of
+ * 256-thread blocks, 1 .. 2^20, each thread moving four 16-byte words per
+ * iteration of a grid-stride loop. */
+int pycles_stream(int mode, const void *src, void *dst, long long bytes, int blocks, void *stream);
+
 /* ---- learned first guess of the saturation adjustment -----------------
  * No counterpart in the reference: its eos_c (thermodynamics_sa.h:17-66)
  * starts the secant from the unsaturated temperature.  The network maps

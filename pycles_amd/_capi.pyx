@@ -114,6 +114,7 @@ cdef extern from "pycles_amd.h":
     int pycles_cloud_stats(const pycles_dims *dims, const double *rho0_half, const void *ql, int precision,
                            double *out, void *workspace, void *stream)
     int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len)
+    int pycles_stream(int mode, const void *src, void *dst, long long nbytes, int blocks, void *stream)
     int pycles_pressure_plan_create(const pycles_dims *dims, const double *rho0_host,
                                     const double *rho0_half_host, int precision,
                                     pycles_pressure_plan **plan)
@@ -454,6 +455,11 @@ def launch_chunks(int kernel, Dims dims):
     cdef int nch = 0, ln = 0
     _check(pycles_launch_chunks(kernel, &dims.d, &nch, &ln))
     return nch, ln
+
+
+def stream_kernel(int mode, src, dst, long long nbytes, int blocks, stream):
+    """HBM calibration kernel: 0 copy, 1 read, 2 fill (+4: non-temporal)."""
+    _check(pycles_stream(mode, _p(src), _p(dst), nbytes, blocks, _p(stream)))
 
 
 def divergence(Dims dims, Ref ref, u, v, w, div, int precision, stream):

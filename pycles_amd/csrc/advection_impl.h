@@ -22,9 +22,23 @@
 // 32-bit vector offset; wrapped horizontal neighbour offsets are computed
 // once per thread, outside the march.
 #pragma once
+#include <utility>
+
 #include "common.h"
 
 namespace adv {
+
+// Calls f(integral_constant<int, Q>) for Q = Q0 .. N-1 while it returns true;
+// returns false as soon as one call does.
+template <int Q, int N, class F>
+__device__ __forceinline__ bool phases(F &f) {
+    if constexpr (Q == N) {
+        return true;
+    } else {
+        if (!f(std::integral_constant<int, Q>{})) return false;
+        return phases<Q + 1, N>(f);
+    }
+}
 
 enum { VS0 = 0, VS1 = 1, VU = 2, VV = 3, VW = 4 };
 
@@ -53,6 +67,25 @@ __device__ __forceinline__ T ld(const T *p, unsigned idx) {
 template <typename T>
 __device__ __forceinline__ T *at(T *p, unsigned idx) {
     return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (size_t)(idx * (unsigned)sizeof(T)));
+}
+
+#ifndef PYCLES_ADV_NT
+#define PYCLES_ADV_NT 1
+#endif
+#ifndef PYCLES_ADV_ZRING
+#define PYCLES_ADV_ZRING 1
+#endif
+// Streamed accesses (read or written once per pass: tendencies, stage
+// copies, new values, window entries no other thread reads).
+template <typename T>
+__device__ __forceinline__ T lds_(const T *p, unsigned idx) {
+    if (PYCLES_ADV_NT) return __builtin_nontemporal_load(at(const_cast<T *>(p), idx));
+    return ld(p, idx);
+}
+template <typename T>
+__device__ __forceinline__ void sts_(T *p, unsigned idx, T v) {
+    if (PYCLES_ADV_NT) __builtin_nontemporal_store(v, at(p, idx));
+    else *at(p, idx) = v;
 }
 
 // tend -= d, or tend = 0 - d when the tendency is known to be zero (same bits).
@@ -206,14 +239,14 @@ __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, un
     const TS dt = (TS)fl.dt;
     TS vn;
     if (RKM == 1) {
-        if (fl.cp[v]) *at((TS *)fl.cp[v], o) = vold;  // NULL: the caller keeps the input as the copy
+        if (fl.cp[v]) sts_((TS *)fl.cp[v], o, vold);  // NULL: the caller keeps the input as the copy
         vn = rk_update<TS, 0>(vold, tn, TS(0), dt);
     } else if (fl.rk == 1) {
         vn = rk_update<TS, 1>(vold, tn, cpv, dt);
     } else {
         vn = rk_update<TS, 2>(vold, tn, cpv, dt);
     }
-    *at(out, o) = vn;
+    sts_(out, o, vn);
     if (!asym) {
         if (c < 2 * gw) *at(out, (unsigned)(2 * gw - 1 - c) * plane + colo) = vn;
         if (c >= nzg - 2 * gw) *at(out, (unsigned)(2 * (nzg - gw) - 1 - c) * plane + colo) = vn;
@@ -230,7 +263,9 @@ __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, un
 // stencils (used after the scalar fluxes) are issued, and this cell's
 // tendencies and stage copies together after the fluxes; all unconditional,
 // so one counted wait covers each group.  The first face (k0 - 1, flux only)
-// is peeled off the loop.
+// is peeled off the loop.  The velocity and scalar windows are register
+// rings as in the y pass below; the scalars' FP64 base window, which is the
+// same for every lane and lives in scalar registers, shifts.
 template <class P, int SO, int MO, int NSC, bool MOM, int RKM>
 __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchunk) {
     using TS = typename P::TS;
@@ -240,6 +275,10 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int NC = 2 * (RM - 1);
     constexpr int NS = NSC > 0 ? NSC : 1;
+    constexpr int WS = 2 * RS, WM = 2 * RM;
+    constexpr int WMAX = (NSC > 0 && WS > WM) || !MOM ? WS : WM;
+    constexpr bool RING = PYCLES_ADV_ZRING && WMAX + 1 <= 8;
+    constexpr int L = RING ? WMAX + 1 : WMAX;
     const uint3 tile = xcd_tile<1>();  // XCD bands of 4-row blocks in y
     const int i = tile.x * 64 + threadIdx.x;
     const int j = tile.y * 4 + threadIdx.y;
@@ -261,10 +300,10 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         cy[m] = (unsigned)wrapi(j + m - (RM - 2), g.ny) * g.nx + i;
     }
     auto base_at = [&](int sc, int lev) { return P::kBase ? (fl.base[sc] ? fl.base[sc][lev] : 0.0) : 0.0; };
-    // windows of face c: level c - (R-1) + m in slot m
-    TC ws[NS][2 * RS];
+    // windows of face c: level c - (R-1) + m in slot m (rings: phase 0)
+    TC rs[NS][L];
     double bw[NS][2 * RS];
-    TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
+    TC ru[L], rv[L], rw[L];
     TC cwx[NC], cwy[NC];  // w along x and y at level c
     int c = k0 - 1;
 #pragma unroll
@@ -272,16 +311,16 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
 #pragma unroll
         for (int m = 0; m < 2 * RS; ++m) {
             const int lev = c - (RS - 1) + m;
-            ws[sc][m] = (TC)ld(ps[sc], lev * plane + col);
+            rs[sc][m] = (TC)ld(ps[sc], lev * plane + col);
             bw[sc][m] = base_at(sc, lev);
         }
     if (MOM) {
 #pragma unroll
         for (int m = 0; m < 2 * RM; ++m) {
             const unsigned o = (unsigned)(c - (RM - 1) + m) * plane + col;
-            wsu[m] = (TC)ld(pu, o);
-            wsv[m] = (TC)ld(pv, o);
-            wsw[m] = (TC)ld(pw, o);
+            ru[m] = (TC)ld(pu, o);
+            rv[m] = (TC)ld(pv, o);
+            rw[m] = (TC)ld(pw, o);
         }
     }
     auto load_centred = [&](int cc) {
@@ -311,27 +350,59 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
             nw = (TC)ld(pw, o);
         }
     };
-    auto advance = [&]() {
+    // the next levels into the rings' free slots (phase PH), or, without
+    // rings, the windows shifted; the base window always shifts
+    auto advance = [&](auto phc) {
+        constexpr int PH = RING ? decltype(phc)::value : 0;
 #pragma unroll
         for (int sc = 0; sc < NSC; ++sc) {
 #pragma unroll
-            for (int m = 0; m < 2 * RS - 1; ++m) {
-                ws[sc][m] = ws[sc][m + 1];
-                bw[sc][m] = bw[sc][m + 1];
-            }
-            ws[sc][2 * RS - 1] = nws[sc];
-            bw[sc][2 * RS - 1] = nbw[sc];
+            for (int m = 0; m < WS - 1; ++m) bw[sc][m] = bw[sc][m + 1];
+            bw[sc][WS - 1] = nbw[sc];
         }
+        if (RING) {
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) rs[sc][(WS + PH) % L] = nws[sc];
+            if (MOM) {
+                ru[(WM + PH) % L] = nu;
+                rv[(WM + PH) % L] = nv;
+                rw[(WM + PH) % L] = nw;
+            }
+        } else {
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) {
+#pragma unroll
+                for (int m = 0; m < WS - 1; ++m) rs[sc][m] = rs[sc][m + 1];
+                rs[sc][WS - 1] = nws[sc];
+            }
+            if (MOM) {
+#pragma unroll
+                for (int m = 0; m < WM - 1; ++m) {
+                    ru[m] = ru[m + 1];
+                    rv[m] = rv[m + 1];
+                    rw[m] = rw[m + 1];
+                }
+                ru[WM - 1] = nu;
+                rv[WM - 1] = nv;
+                rw[WM - 1] = nw;
+            }
+        }
+    };
+    // the windows of the face at phase PH
+    TC ws[NS][WS], wsu[WM], wsv[WM], wsw[WM];
+    auto windows = [&](auto phc) {
+        constexpr int PH = RING ? decltype(phc)::value : 0;
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc)
+#pragma unroll
+            for (int m = 0; m < WS; ++m) ws[sc][m] = rs[sc][(m + PH) % L];
         if (MOM) {
 #pragma unroll
-            for (int m = 0; m < 2 * RM - 1; ++m) {
-                wsu[m] = wsu[m + 1];
-                wsv[m] = wsv[m + 1];
-                wsw[m] = wsw[m + 1];
+            for (int m = 0; m < WM; ++m) {
+                wsu[m] = ru[(m + PH) % L];
+                wsv[m] = rv[(m + PH) % L];
+                wsw[m] = rw[(m + PH) % L];
             }
-            wsu[2 * RM - 1] = nu;
-            wsv[2 * RM - 1] = nv;
-            wsw[2 * RM - 1] = nw;
         }
     };
     SFlux Fs[NS], Fs_prev[NS];
@@ -356,13 +427,19 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     const double dzi = g.dxi[2];
     issue_next(c);
     load_centred(c);
+    windows(std::integral_constant<int, 0>{});
     fluxes(c, Fs_prev, Fu_prev, Fv_prev, Fw_prev);
-    advance();
-    for (c = k0; c < k1; ++c) {
-        const unsigned o = (unsigned)c * plane + col;
-        issue_next(c);
-        load_centred(c);
-        fluxes(c, Fs, Fu, Fv, Fw);
+    advance(std::integral_constant<int, 0>{});
+    // faces at phases 1, 2, .., L-1, 0, 1, ...
+    auto face = [&](auto phc) {
+        // the level as an opaque scalar: offsets affine in the loop counter
+        // would become one induction register per unrolled face and address
+        const int cr = __builtin_amdgcn_readfirstlane(c);
+        const unsigned o = (unsigned)cr * plane + col;
+        issue_next(cr);
+        load_centred(cr);
+        windows(phc);
+        fluxes(cr, Fs, Fu, Fv, Fw);
         // this cell's tendencies and stage copies, all read before any store
         // (the compiler must assume the output arrays alias them); read after
         // the flux arithmetic, which keeps them out of its register peak
@@ -370,11 +447,11 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
             if (v < NSC || (MOM && v >= VU)) {
-                told[v] = ld((const TS *)fl.t[v], o);
-                if (RKM == 2) cpv[v] = ld((const TS *)fl.cp[v], o);
+                told[v] = lds_((const TS *)fl.t[v], o);
+                if (RKM == 2) cpv[v] = lds_((const TS *)fl.cp[v], o);
             }
         }
-        const double ah = pr.alpha0_half[c];
+        const double ah = pr.alpha0_half[cr];
         // the final tendencies, with an uncontracted subtraction so the
         // stored (separate RK3) and fused paths round identically
         TS tn[5] = {};
@@ -384,20 +461,20 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         if (MOM) {
             tn[VU] = add_rn(told[VU], -P::mom_div(Fu, Fu_prev, ah, dzi));
             tn[VV] = add_rn(told[VV], -P::mom_div(Fv, Fv_prev, ah, dzi));
-            tn[VW] = add_rn(told[VW], -P::mom_div(Fw, Fw_prev, pr.alpha0[c], dzi));
+            tn[VW] = add_rn(told[VW], -P::mom_div(Fw, Fw_prev, pr.alpha0[cr], dzi));
         }
         if (RKM == 0) {
 #pragma unroll
             for (int v = 0; v < 5; ++v)
-                if (v < NSC || (MOM && v >= VU)) *at((TS *)fl.t[v], o) = tn[v];
+                if (v < NSC || (MOM && v >= VU)) sts_((TS *)fl.t[v], o, tn[v]);
         } else {
 #pragma unroll
             for (int sc = 0; sc < NSC; ++sc)
-                rk_store<TS, RKM>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn[sc], cpv[sc], false);
+                rk_store<TS, RKM>(fl, sc, o, col, plane, cr, g.gw, g.nzg, (TS)ws[sc][RS - 1], tn[sc], cpv[sc], false);
             if (MOM) {
-                rk_store<TS, RKM>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)wsu[RM - 1], tn[VU], cpv[VU], false);
-                rk_store<TS, RKM>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)wsv[RM - 1], tn[VV], cpv[VV], false);
-                rk_store<TS, RKM>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)wsw[RM - 1], tn[VW], cpv[VW], true);
+                rk_store<TS, RKM>(fl, VU, o, col, plane, cr, g.gw, g.nzg, (TS)wsu[RM - 1], tn[VU], cpv[VU], false);
+                rk_store<TS, RKM>(fl, VV, o, col, plane, cr, g.gw, g.nzg, (TS)wsv[RM - 1], tn[VV], cpv[VV], false);
+                rk_store<TS, RKM>(fl, VW, o, col, plane, cr, g.gw, g.nzg, (TS)wsw[RM - 1], tn[VW], cpv[VW], true);
             }
         }
 #pragma unroll
@@ -405,7 +482,16 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
         Fu_prev = Fu;
         Fv_prev = Fv;
         Fw_prev = Fw;
-        advance();
+        advance(phc);
+    };
+    c = k0;
+    if (c < k1) {
+        auto step = [&](auto q) {
+            face(std::integral_constant<int, (1 + decltype(q)::value) % L>{});
+            return ++c < k1;
+        };
+        while (phases<0, RING ? L : 1>(step)) {
+        }
     }
 }
 
@@ -415,6 +501,12 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
 // Pipelined like the z pass: the next row of every window and this face's
 // centred v stencils are loaded at the top of each face, the tendencies
 // together after the fluxes; the first face is peeled off the loop.
+// The stencil windows are rings of L = max(2 RS, 2 RM) + 1 registers: at
+// phase p (face count mod L) window slot m is ring slot (m + p) % L, and the
+// next row loads into the slot that the oldest row has left, so the march,
+// unrolled by L faces, moves no register (a shifted window costs one copy per
+// entry and face).  Above L = 8 (WENO7 and higher) the unrolled march would
+// outgrow the instruction cache, and the windows shift instead (L = W).
 template <class P, int SO, int MO, int NSC, bool MOM>
 __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchunk) {
     using TS = typename P::TS;
@@ -424,6 +516,10 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
     constexpr int RM = Interp<MO, TC>::R;
     constexpr int NC = 2 * (RM - 1);
     constexpr int NS = NSC > 0 ? NSC : 1;
+    constexpr int WS = 2 * RS, WM = 2 * RM;
+    constexpr int WMAX = (NSC > 0 && WS > WM) || !MOM ? WS : WM;
+    constexpr bool RING = WMAX + 1 <= 8;
+    constexpr int L = RING ? WMAX + 1 : WMAX;
     const uint3 tile = xcd_tile<1>();  // XCD bands of 4-level blocks in z
     const int i = tile.x * 64 + threadIdx.x;
     const int kk = tile.y * 4 + threadIdx.y;
@@ -456,30 +552,42 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
         bw[sc][0] = P::kBase ? (fl.base[sc] ? fl.base[sc][k] : 0.0) : 0.0;
         brho[sc] = bw[sc][0] * rh;
     }
-    TC ws[NS][2 * RS];
-    TC wsu[2 * RM], wsv[2 * RM], wsw[2 * RM];
+    // rings (phase 0: window slot m in ring slot m)
+    TC rs[NS][L];
+    TC ru[L], rv[L], rw[L];
     TC cvx[NC], cvz[NC];  // v along x and z at row c
     int c = j0 - 1;
 #pragma unroll
     for (int sc = 0; sc < NSC; ++sc)
 #pragma unroll
-        for (int m = 0; m < 2 * RS; ++m)
-            ws[sc][m] = (TC)ld(ps[sc], lev + (unsigned)wrapi(c - (RS - 1) + m, g.ny) * nx);
+        for (int m = 0; m < WS; ++m)
+            rs[sc][m] = (TC)ld(ps[sc], lev + (unsigned)wrapi(c - (RS - 1) + m, g.ny) * nx);
     if (MOM) {
 #pragma unroll
-        for (int m = 0; m < 2 * RM; ++m) {
+        for (int m = 0; m < WM; ++m) {
             const unsigned o = lev + (unsigned)wrapi(c - (RM - 1) + m, g.ny) * nx;
-            wsu[m] = (TC)ld(pu, o);
-            wsv[m] = (TC)ld(pv, o);
-            wsw[m] = (TC)ld(pw, o);
+            ru[m] = (TC)ld(pu, o);
+            rv[m] = (TC)ld(pv, o);
+            rw[m] = (TC)ld(pw, o);
         }
     }
     // rows (wrapped) of the entries that enter the windows at face c + 1,
     // c + 1 + R, and of the current face
     int jn_s = wrapi(c + 1 + RS, g.ny), jn_m = wrapi(c + 1 + RM, g.ny);
     int cw = c < 0 ? c + g.ny : c;
-    TC nws[NS], nu = 0, nv = 0, nw = 0;
-    auto issue_next = [&]() {
+    SFlux Fs[NS], Fs_prev[NS];
+    TC Fu = 0, Fv = 0, Fw = 0, Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
+    // one face at ring phase PH (a compile-time constant when RING): the next
+    // rows into the free ring slots, the centred stencils, the fluxes and,
+    // unless FIRST, the tendencies; then the row counters advance (and, when
+    // the windows do not rotate, they shift)
+    auto face = [&](auto phc, auto firstc) {
+        constexpr int PH = RING ? decltype(phc)::value : 0;
+        constexpr bool FIRST = decltype(firstc)::value;
+        auto slot = [](int m) { return (m + PH) % L; };
+        const unsigned orow = lev + (unsigned)cw * nx;  // (i, c, k)
+        // next rows: window slot W of this face (slot W - 1 of the next)
+        TC nws[NS], nu = 0, nv = 0, nw = 0;
 #pragma unroll
         for (int sc = 0; sc < NSC; ++sc) nws[sc] = (TC)ld(ps[sc], lev + (unsigned)jn_s * nx);
         if (MOM) {
@@ -488,8 +596,15 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
             nv = (TC)ld(pv, o);
             nw = (TC)ld(pw, o);
         }
-    };
-    auto load_centred = [&](unsigned orow) {
+        if (RING) {
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) rs[sc][slot(WS)] = nws[sc];
+            if (MOM) {
+                ru[slot(WM)] = nu;
+                rv[slot(WM)] = nv;
+                rw[slot(WM)] = nw;
+            }
+        }
         if (MOM) {
 #pragma unroll
             for (int m = 0; m < NC; ++m) {
@@ -497,32 +612,27 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
                 cvz[m] = (TC)ld(pv, orow + dzo[m]);
             }
         }
-    };
-    auto advance = [&]() {
+        // keep the loads above the flux arithmetic (the scheduler would sink
+        // them next to their first use)
+        if (!FIRST) __builtin_amdgcn_sched_barrier(0);
+        // this face's windows
+        TC ws[NS][WS], wsu[WM], wsv[WM], wsw[WM];
 #pragma unroll
-        for (int sc = 0; sc < NSC; ++sc) {
+        for (int sc = 0; sc < NSC; ++sc)
 #pragma unroll
-            for (int m = 0; m < 2 * RS - 1; ++m) ws[sc][m] = ws[sc][m + 1];
-            ws[sc][2 * RS - 1] = nws[sc];
-        }
+            for (int m = 0; m < WS; ++m) ws[sc][m] = rs[sc][slot(m)];
         if (MOM) {
 #pragma unroll
-            for (int m = 0; m < 2 * RM - 1; ++m) {
-                wsu[m] = wsu[m + 1];
-                wsv[m] = wsv[m + 1];
-                wsw[m] = wsw[m + 1];
+            for (int m = 0; m < WM; ++m) {
+                wsu[m] = ru[slot(m)];
+                wsv[m] = rv[slot(m)];
+                wsw[m] = rw[slot(m)];
             }
-            wsu[2 * RM - 1] = nu;
-            wsv[2 * RM - 1] = nv;
-            wsw[2 * RM - 1] = nw;
         }
-        cw = cw + 1 == g.ny ? 0 : cw + 1;
-        jn_s = jn_s + 1 == g.ny ? 0 : jn_s + 1;
-        jn_m = jn_m + 1 == g.ny ? 0 : jn_m + 1;
-    };
-    SFlux Fs[NS], Fs_prev[NS];
-    TC Fu = 0, Fv = 0, Fw = 0, Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
-    auto fluxes = [&](unsigned orow, SFlux *F, TC &Fu_, TC &Fv_, TC &Fw_) {
+        SFlux *F = FIRST ? Fs_prev : Fs;
+        TC &Fu_ = FIRST ? Fu_prev : Fu;
+        TC &Fv_ = FIRST ? Fv_prev : Fv;
+        TC &Fw_ = FIRST ? Fw_prev : Fw;
         const TC V = MOM ? wsv[RM - 1] : (TC)ld(pv, orow);
 #pragma unroll
         for (int sc = 0; sc < NSC; ++sc) F[sc] = P::template scal_flux<SO, false>(ws[sc], bw[sc], V, rh);
@@ -536,41 +646,61 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
             Fv_ = P::template mom_flux<MO>(wsv, Interp<MO, TC>::centred(q), rh);
             Fw_ = P::template mom_flux<MO>(wsw, Interp<MO, TC>::centred(cvz), r0);  // v along z
         }
-    };
-    {
-        const unsigned orow = lev + (unsigned)cw * nx;
-        issue_next();
-        load_centred(orow);
-        fluxes(orow, Fs_prev, Fu_prev, Fv_prev, Fw_prev);
-        advance();
-    }
-    for (c = j0; c < j1; ++c) {
-        const unsigned orow = lev + (unsigned)cw * nx;  // (i, c, k)
-        issue_next();
-        load_centred(orow);
-        // keep the loads above the flux arithmetic (the scheduler would sink
-        // them next to their first use)
-        __builtin_amdgcn_sched_barrier(0);
-        fluxes(orow, Fs, Fu, Fv, Fw);
-        // tendencies, all read before any store (see the z pass)
-        TS told[5] = {};
+        if (!FIRST) {
+            // tendencies, all read before any store (see the z pass)
+            TS told[5] = {};
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
-            if (v < NSC || (MOM && v >= VU)) told[v] = ld((const TS *)fl.t[v], orow);
+            for (int v = 0; v < 5; ++v)
+                if (v < NSC || (MOM && v >= VU)) told[v] = lds_((const TS *)fl.t[v], orow);
 #pragma unroll
-        for (int sc = 0; sc < NSC; ++sc)
-            *at((TS *)fl.t[sc], orow) = told[sc] - P::template scal_div<false>(Fs[sc], Fs_prev[sc], ah, dyi, brho[sc]);
-        if (MOM) {
-            *at((TS *)fl.t[VU], orow) = told[VU] - P::mom_div(Fu, Fu_prev, ah, dyi);
-            *at((TS *)fl.t[VV], orow) = told[VV] - P::mom_div(Fv, Fv_prev, ah, dyi);
-            *at((TS *)fl.t[VW], orow) = told[VW] - P::mom_div(Fw, Fw_prev, a0, dyi);
+            for (int sc = 0; sc < NSC; ++sc)
+                sts_((TS *)fl.t[sc], orow,
+                     told[sc] - P::template scal_div<false>(Fs[sc], Fs_prev[sc], ah, dyi, brho[sc]));
+            if (MOM) {
+                sts_((TS *)fl.t[VU], orow, (TS)(told[VU] - P::mom_div(Fu, Fu_prev, ah, dyi)));
+                sts_((TS *)fl.t[VV], orow, (TS)(told[VV] - P::mom_div(Fv, Fv_prev, ah, dyi)));
+                sts_((TS *)fl.t[VW], orow, (TS)(told[VW] - P::mom_div(Fw, Fw_prev, a0, dyi)));
+            }
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) Fs_prev[sc] = Fs[sc];
+            Fu_prev = Fu;
+            Fv_prev = Fv;
+            Fw_prev = Fw;
         }
+        if (!RING) {
 #pragma unroll
-        for (int sc = 0; sc < NSC; ++sc) Fs_prev[sc] = Fs[sc];
-        Fu_prev = Fu;
-        Fv_prev = Fv;
-        Fw_prev = Fw;
-        advance();
+            for (int sc = 0; sc < NSC; ++sc) {
+#pragma unroll
+                for (int m = 0; m < WS - 1; ++m) rs[sc][m] = rs[sc][m + 1];
+                rs[sc][WS - 1] = nws[sc];
+            }
+            if (MOM) {
+#pragma unroll
+                for (int m = 0; m < WM - 1; ++m) {
+                    ru[m] = ru[m + 1];
+                    rv[m] = rv[m + 1];
+                    rw[m] = rw[m + 1];
+                }
+                ru[WM - 1] = nu;
+                rv[WM - 1] = nv;
+                rw[WM - 1] = nw;
+            }
+        }
+        cw = cw + 1 == g.ny ? 0 : cw + 1;
+        jn_s = jn_s + 1 == g.ny ? 0 : jn_s + 1;
+        jn_m = jn_m + 1 == g.ny ? 0 : jn_m + 1;
+    };
+    face(std::integral_constant<int, 0>{}, std::true_type{});
+    c = j0;
+    if (c < j1) {
+        // faces at phases 1, 2, .., L-1, 0, 1, ...: each returns whether
+        // another face follows
+        auto step = [&](auto q) {
+            face(std::integral_constant<int, (1 + decltype(q)::value) % L>{}, std::false_type{});
+            return ++c < j1;
+        };
+        while (phases<0, RING ? L : 1>(step)) {
+        }
     }
 }
 

@@ -40,3 +40,22 @@ def rel_err(a, b):
     if scale == 0.0:
         return float(np.max(np.abs(a)))
     return float(np.max(np.abs(a - b)) / scale)
+
+
+def ulp_perturbed(fields, name="s"):
+    """The state with every nonzero value of one field moved up by one FP64
+    ulp: the smallest perturbation a rounding difference makes."""
+    out = dict(fields)
+    f = np.asarray(fields[name])
+    out[name] = np.where(f != 0.0, np.nextafter(f, np.inf), 0.0)
+    return out
+
+
+def floor_tolerance(err_floor, tol=1e-12, factor=10.0):
+    """Per-field tolerance of a multi-step comparison: tol, or `factor` times
+    how far the oracle moves from a one-ulp perturbation of its own start
+    over the same steps (where that is larger, rounding differences between
+    two correct evaluations are amplified by the dynamics: near-zero
+    arguments of the Smagorinsky stability function's square root, secant
+    stops, upwind switches)."""
+    return {n: max(tol, factor * v) for n, v in err_floor.items()}

@@ -14,12 +14,12 @@ import numpy as np
 import pytest
 import torch
 
-from tests.helpers import oracle_sim, rel_err
+from tests.helpers import floor_tolerance, oracle_sim, rel_err, ulp_perturbed
 
 pytestmark = pytest.mark.gpu
 
 
-def _bomex(nx=24, ny=16, nz=75, precision="f64", seed=61, dt=10.0):
+def _bomex(nx=32, ny=16, nz=75, precision="f64", seed=61, dt=10.0):
     from pycles_amd import cases
     from pycles_amd.Grid import Grid
     from pycles_amd.ReferenceState import ReferenceState
@@ -36,6 +36,9 @@ def _bomex(nx=24, ny=16, nz=75, precision="f64", seed=61, dt=10.0):
         f = fields[n].copy()
         f[d.gw:d.gw + d.n[2]] += amp * rng.standard_normal((d.n[2], d.n[1], d.n[0]))
         fields[n] = cases.fill_z_ghosts(f, d.gw, -1.0 if n == "w" else 1.0)
+    # divergence-free, as the stepping assumes (a divergent start gives the
+    # first stage flux-form scalar tendencies of order s div(u))
+    fields["u"], fields["v"], fields["w"] = cases.project_initial(Gr, RS, fields["u"], fields["v"], fields["w"])
     return nl, Gr, RS, Th, fields
 
 
@@ -50,8 +53,8 @@ def _sim(nl, Gr, RS, Th, fields):
     return Simulation3d().initialize(nl, Gr=Gr, Ref=RS, Th=Th, fields=fields)
 
 
-def _close(g, o, tol=1e-12):
-    scale = np.abs(o).max()
+def _close(g, o, tol=1e-12, scale=0.0):
+    scale = max(np.abs(o).max(), scale)
     return np.abs(g - o).max() <= tol * max(scale, 1e-300)
 
 
@@ -72,6 +75,8 @@ def test_bomex_operators(gpu):
     assert np.array_equal(sim.Fo.scheme.profiles_host.reshape(-1), osim.forcing_prof)
     assert np.array_equal(sim.Damping.scheme.gamma_z_host, osim.gamma_z)
     assert np.array_equal(sim.Damping.scheme.gamma_zhalf_host, osim.gamma_zhalf)
+    gamma_max = max(np.abs(osim.gamma_z).max(), np.abs(osim.gamma_zhalf).max())
+    assert gamma_max == pytest.approx(nl["damping"]["Rayleigh"]["gamma_r"], rel=0.05)
     for step, (gpu_op, orc_op) in enumerate((
             (lambda: sim.Sur.update(sim.Gr, sim.Ref, sim.PV, sim.DV, sim.Pa, sim.TS), osim.surface_update),
             (lambda: sim.Damping.update(sim.Gr, sim.Ref, sim.PV, sim.DV, sim.Pa), osim.damping_update),
@@ -84,7 +89,13 @@ def test_bomex_operators(gpu):
         for n in osim.pv_names:
             g = sim.PV.get_tendency_array(n, Gr)[sl]
             o = osim.pt(n)[sl]
-            assert _close(g, o), (step, n, np.abs(g - o).max(), np.abs(o).max())
+            # the damping tendency gamma (phi - <phi>) is the difference of two
+            # terms of size gamma |phi|; above the noise layer phi is uniform
+            # along each level and the tendency is the rounding of the level
+            # mean (the device sums in a tree, the reference serially), so it
+            # is compared at the scale of those terms
+            scale = gamma_max * np.abs(osim.pv(n)[sl]).max() if step == 1 else 0.0
+            assert _close(g, o, scale=scale), (step, n, np.abs(g - o).max(), np.abs(o).max(), scale)
     # the surface operator touched the first level, forcing all, damping the top
     fl = sim.Sur.flux2d.cpu().numpy()
     of = osim.surface_flux2d.reshape(8, Gr.dims.nlg[0], Gr.dims.nlg[1])[:, gw:-gw, gw:-gw].transpose(0, 2, 1)
@@ -98,16 +109,28 @@ def test_bomex_steps_against_the_oracle(gpu):
     sim.Th.max_iter = 0
     sim.run(max_steps=2)
     torch.cuda.synchronize()
-    osim = _oracle(Gr, RS, Th, fields, nl, dt=10.0, dt_max=10.0, cfl_limit=0.7)
-    osim.begin_run()
-    osim.step()
-    osim.step()
-    errs = {n: rel_err(sim.PV.get_device_layout(n), Gr.to_device_layout(osim.pv(n))) for n in osim.pv_names}
-    for n in ("temperature", "ql", "viscosity", "dynamic_pressure"):
-        errs[n] = rel_err(sim.DV.get_device_layout(n), Gr.to_device_layout(osim.dv[n]))
-    print("2 FP64 BOMEX steps vs oracle", errs, "t", sim.TS.t, osim.t, "dt", sim.TS.dt, osim.dt)
+    runs = {}
+    for key, start in (("oracle", fields), ("oracle_ulp", ulp_perturbed(fields))):
+        o = _oracle(Gr, RS, Th, start, nl, dt=10.0, dt_max=10.0, cfl_limit=0.7)
+        o.begin_run()
+        o.step()
+        o.step()
+        runs[key] = o
+    osim = runs["oracle"]
+
+    def errs_of(get_pv, get_dv):
+        e = {n: rel_err(get_pv(n), Gr.to_device_layout(osim.pv(n))) for n in osim.pv_names}
+        for n in ("temperature", "ql", "viscosity", "dynamic_pressure"):
+            e[n] = rel_err(get_dv(n), Gr.to_device_layout(osim.dv[n]))
+        return e
+    errs = errs_of(sim.PV.get_device_layout, sim.DV.get_device_layout)
+    o2 = runs["oracle_ulp"]
+    floor = errs_of(lambda n: Gr.to_device_layout(o2.pv(n)), lambda n: Gr.to_device_layout(o2.dv[n]))
+    tol = floor_tolerance(floor)
+    print("2 FP64 BOMEX steps vs oracle", errs, "floor (1 ulp of s)", floor, "t", sim.TS.t, osim.t,
+          "dt", sim.TS.dt, osim.dt)
     assert sim.TS.t == osim.t and sim.TS.dt == pytest.approx(osim.dt, rel=1e-12)
-    assert max(errs.values()) <= 1e-12, errs
+    assert all(errs[n] <= tol[n] for n in errs), (errs, tol)
 
 
 def test_bomex_fp32_tracks_fp64(gpu):
