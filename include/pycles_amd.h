@@ -313,6 +313,18 @@ size_t pycles_horizontal_stats_workspace_bytes(const pycles_dims *dims);
 int pycles_horizontal_stats(const pycles_dims *dims, const void *a, const double *base_a, const void *b,
                             const double *base_b, const void *mask, int precision, double *out,
                             void *workspace, void *stream);
+/* pycles_horizontal_stats without a workspace argument, for callers that
+ * cannot allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * partials live in a device array inside the library (1.5 MiB per GPU, held
+ * until the process ends), so nzg = nz + 2 gw must be <= 2048
+ * (PYCLES_ERR_UNSUPPORTED otherwise).  The call allocates nothing.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other host threads) race.  Results are bitwise those of
+ * pycles_horizontal_stats. */
+int pycles_horizontal_stats_noalloc(const pycles_dims *dims, const void *a, const double *base_a, const void *b,
+                                    const double *base_b, const void *mask, int precision, double *out,
+                                    void *stream);
 /* The fields of ThermodynamicsSA.stats_io (ThermodynamicsSA.pyx:256-389) at
  * every stored point: thetas = thetas_c(s, qt), theta = T / exner(p0_half),
  * rh = pv_c(p0_half, qt, qv) / pv*(T); outputs in the field precision, any

@@ -11,6 +11,8 @@
 // of the level in a fixed order into FP64 partials, and a second kernel adds
 // the partials in split order, so the result does not depend on scheduling.
 // The partials live in a caller-provided workspace.
+#include <mutex>
+
 #include "common.h"
 #include "thermo_funcs.h"
 
@@ -280,6 +282,58 @@ extern "C" int pycles_horizontal_stats(const pycles_dims *dims, const void *a, c
     }
     const Geo g = make_geo(dims);
     return stats_launch(g, a, base_a, b, base_b, mask, precision, out, (double *)workspace, (hipStream_t)stream);
+}
+
+// No-allocation variant of pycles_horizontal_stats for callers that cannot
+// allocate device memory.  Its partials live in a file-scope device array,
+// one copy per GPU, whose address each device resolves once (with that
+// device current) by hipGetSymbolAddress; host code passes only that
+// address, never the array's name.  Every (level, split) slot is written by
+// stats_partial_kernel before stats_final_kernel reads it, so nothing needs
+// clearing between calls.  All calls on one GPU share the array: calls that
+// overlap on one GPU (two streams, two host threads) race.
+#define STATS_NOALLOC_MAX_LEVELS 2048
+#define STATS_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) double scratch[(size_t)STATS_NOALLOC_MAX_LEVELS * STAT_SPLITS * STAT_NQ];
+static double *g_scratch_dev[STATS_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int stats_scratch(double **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= STATS_NOALLOC_MAX_DEVICES) {
+        pycles_set_error("horizontal_stats_noalloc: no usable current device (%d, %s)", dev, hipGetErrorString(e));
+        return PYCLES_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            pycles_set_error("horizontal_stats_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+            return PYCLES_ERR_HIP;
+        }
+        g_scratch_dev[dev] = (double *)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return PYCLES_OK;
+}
+
+extern "C" int pycles_horizontal_stats_noalloc(const pycles_dims *dims, const void *a, const double *base_a,
+                                               const void *b, const double *base_b, const void *mask,
+                                               int precision, double *out, void *stream) {
+    int rc = stats_args(dims, a, precision, out);
+    if (rc) return rc;
+    const Geo g = make_geo(dims);
+    if (g.nzg > STATS_NOALLOC_MAX_LEVELS) {
+        pycles_set_error("horizontal_stats_noalloc: %d levels (with ghosts) exceed the scratch limit of %d",
+                         g.nzg, STATS_NOALLOC_MAX_LEVELS);
+        return PYCLES_ERR_UNSUPPORTED;
+    }
+    double *partial = nullptr;
+    rc = stats_scratch(&partial);
+    if (rc) return rc;
+    return stats_launch(g, a, base_a, b, base_b, mask, precision, out, partial, (hipStream_t)stream);
 }
 
 extern "C" size_t pycles_cloud_stats_workspace_bytes(const pycles_dims *dims) {
