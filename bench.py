@@ -162,16 +162,47 @@ def perturbed_fields(fields, kind):
     return out
 
 
-def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True):
+class FP32Storage:
+    """After every step, round an FP64 run's prognostic state to what the FP32
+    form stores: u, v, w to FP32, s and qt to their level profile (the
+    initial horizontal mean) plus an FP32-rounded deviation.  The run then
+    carries FP32 storage's rounding, step after step, with FP64 arithmetic:
+    how far it moves from the plain FP64 run is the floor of any comparison
+    of an FP32-storage form with FP64."""
+
+    def __init__(self, sim, fields):
+        import torch
+        self.sim = sim
+        self.base = {n: torch.from_numpy(np.asarray(fields[n], dtype=np.float64).mean(axis=(1, 2))).to(
+            sim.PV.values.device)[:, None, None] for n in ("s", "qt")}
+
+    def round(self):
+        PV = self.sim.PV
+        for n in PV.index_name:
+            f = PV.field(n)
+            if n in self.base:
+                b = self.base[n]
+                f.copy_(b + (f - b).float().double())
+            else:
+                f.copy_(f.float().double())
+
+
+def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True, floor_kinds=("ulp64", "fp32", "storage")):
     """FP32 + learned against FP64 from the same state after 1 and nsteps
     steps; with `floors`, also FP64 runs from the perturbed states of
-    `perturbed_fields` against the same FP64 run (their noise floors)."""
+    `perturbed_fields` and an FP64 run rounded to FP32 storage after every
+    step (FP32Storage) against the same FP64 run (their noise floors)."""
     import torch
     runs = {"fp32_learned": make_sim(nl, Gr, RS, Th, fields, "f32", learned),
             "fp64": make_sim(nl, Gr, RS, Th, fields, "f64")}
+    storage = None
     if floors:
         for kind in ("ulp64", "fp32"):
-            runs["fp64_from_" + kind] = make_sim(nl, Gr, RS, Th, perturbed_fields(fields, kind), "f64")
+            if kind in floor_kinds:
+                runs["fp64_from_" + kind] = make_sim(nl, Gr, RS, Th, perturbed_fields(fields, kind), "f64")
+        if "storage" in floor_kinds:
+            runs["fp64_fp32_storage"] = make_sim(nl, Gr, RS, Th, perturbed_fields(fields, "fp32"), "f64")
+            storage = FP32Storage(runs["fp64_fp32_storage"], fields)
     out = {"errors_vs_fp64": {}, "noise_floor_fp64": {}}
     for name, sim in runs.items():
         sim.Th.mlp = learned if name == "fp32_learned" else None
@@ -180,12 +211,14 @@ def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True):
         for name, sim in runs.items():
             sim.Th.mlp = learned if name == "fp32_learned" else None
             sim.step()
+        if storage is not None:
+            storage.round()
         if n == 1 or n == nsteps:
             torch.cuda.synchronize()
             key = f"after_{n}_steps"
             out["errors_vs_fp64"][key] = field_errors(runs["fp32_learned"], runs["fp64"])
             for name in runs:
-                if name.startswith("fp64_from_"):
+                if name.startswith("fp64_from_") or name == "fp64_fp32_storage":
                     out["noise_floor_fp64"].setdefault(name, {})[key] = field_errors(runs[name], runs["fp64"])
             e = out["errors_vs_fp64"][key]
             log(f"[bench] errors after {n} steps: T {e['temperature']} qt {e['qt']}")
@@ -195,15 +228,26 @@ def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True):
     return out
 
 
-def error_gate(errors, tol=1e-5):
-    """The metric's gate: T and qt within tol (max relative) of FP64."""
+def error_gate(errors, tol=1e-5, floor=None):
+    """The metric's gate: T and qt within tol (max relative) of FP64.  With
+    `floor` (the FP32-storage run's errors), also the gate against what FP32
+    storage alone gives: within max(tol, 2 x floor)."""
     out = {"tolerance": tol, "fields": ["temperature", "qt"], "failing": []}
+    if floor:
+        out["floor_gate"] = {"rule": "max_rel <= max(tol, 2 x FP64 rounded to FP32 storage every step)",
+                             "failing": []}
     for key, e in errors.items():
         for f in out["fields"]:
             v = e[f]["max_rel"]
             if not v <= tol:
                 out["failing"].append(f"{f} {key}: {v:.3g}")
+            if floor and key in floor:
+                lim = max(tol, 2.0 * floor[key][f]["max_rel"])
+                if not v <= lim:
+                    out["floor_gate"]["failing"].append(f"{f} {key}: {v:.3g} > {lim:.3g}")
     out["pass"] = not out["failing"]
+    if floor:
+        out["floor_gate"]["pass"] = not out["floor_gate"]["failing"]
     return out
 
 
@@ -499,7 +543,8 @@ def main():
         log(f"[bench] error study: {args.error_steps} steps")
         es = error_study(nl, Gr, RS, Th, fields, guess, args.error_steps, floors=not args.no_floor)
         out["errors_vs_fp64"] = es["errors_vs_fp64"]
-        out["error_gate"] = error_gate(es["errors_vs_fp64"])
+        out["error_gate"] = error_gate(es["errors_vs_fp64"],
+                                       floor=es["noise_floor_fp64"].get("fp64_fp32_storage"))
         if es["noise_floor_fp64"]:
             out["noise_floor_fp64"] = es["noise_floor_fp64"]
         torch.cuda.empty_cache()

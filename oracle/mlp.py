@@ -70,13 +70,18 @@ def bf16_round(a):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+# tanh(x) = 1 - 2 / (2^(TANH_C x) + 1); the packed hidden-layer weights are
+# TANH_C W2 rounded to bf16 (FP32 value of 2 log2(e))
+TANH_C = float(np.float32(2.8853900817779268))
+
+
 def infer_bf16(p, x):
-    """What the packed inference computes: W1, W2, W3 and the first hidden
-    activations rounded to bf16, biases FP32, exact tanh, float64 sums.  (The
-    device scales W2 by 64 before rounding; a power of two leaves the rounded
-    bits unchanged.  Its tanh is a 1024-interval quadratic table within 1.3e-6.)"""
+    """What the packed inference computes: W1, TANH_C W2 (then divided by
+    TANH_C), W3 and the first hidden activations rounded to bf16, biases FP32,
+    exact tanh, float64 sums."""
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, dtype=np.float32))
-    W1, W2, W3 = bf16_round(W1), bf16_round(W2), bf16_round(W3)
+    W1, W3 = bf16_round(W1), bf16_round(W3)
+    W2 = bf16_round(np.float32(TANH_C) * W2.astype(np.float32)) / TANH_C
     h1 = bf16_round(np.tanh(x @ W1.T + b1))
     h2 = np.tanh(h1 @ W2.T + b2)
     return h2 @ W3 + b3

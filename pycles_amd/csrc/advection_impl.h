@@ -69,23 +69,16 @@ __device__ __forceinline__ T *at(T *p, unsigned idx) {
     return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + (size_t)(idx * (unsigned)sizeof(T)));
 }
 
-#ifndef PYCLES_ADV_NT
-#define PYCLES_ADV_NT 1
-#endif
-#ifndef PYCLES_ADV_ZRING
-#define PYCLES_ADV_ZRING 1
-#endif
-// Streamed accesses (read or written once per pass: tendencies, stage
-// copies, new values, window entries no other thread reads).
+// Streamed accesses, non-temporal: the tendencies, stage copies and new
+// values that a pass reads or writes once (the field windows, which
+// neighbouring lanes and blocks re-read, stay ordinary loads).
 template <typename T>
 __device__ __forceinline__ T lds_(const T *p, unsigned idx) {
-    if (PYCLES_ADV_NT) return __builtin_nontemporal_load(at(const_cast<T *>(p), idx));
-    return ld(p, idx);
+    return __builtin_nontemporal_load(at(const_cast<T *>(p), idx));
 }
 template <typename T>
 __device__ __forceinline__ void sts_(T *p, unsigned idx, T v) {
-    if (PYCLES_ADV_NT) __builtin_nontemporal_store(v, at(p, idx));
-    else *at(p, idx) = v;
+    __builtin_nontemporal_store(v, at(p, idx));
 }
 
 // tend -= d, or tend = 0 - d when the tendency is known to be zero (same bits).
@@ -277,7 +270,7 @@ __global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchu
     constexpr int NS = NSC > 0 ? NSC : 1;
     constexpr int WS = 2 * RS, WM = 2 * RM;
     constexpr int WMAX = (NSC > 0 && WS > WM) || !MOM ? WS : WM;
-    constexpr bool RING = PYCLES_ADV_ZRING && WMAX + 1 <= 8;
+    constexpr bool RING = WMAX + 1 <= 8;
     constexpr int L = RING ? WMAX + 1 : WMAX;
     const uint3 tile = xcd_tile<1>();  // XCD bands of 4-row blocks in y
     const int i = tile.x * 64 + threadIdx.x;

@@ -336,14 +336,16 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ P, 
     const int t = threadIdx.x;
     if (t == 0) {
         w->nm = *nmp;
-        w->b3 = P[OB3];
+        float b3s = P[OB3];
+        for (int u = 0; u < NH; ++u) b3s += bf16_round(P[OW3 + u]);
+        w->b3s = b3s;
     }
     for (int q = t; q < 2 * 64; q += blockDim.x) {
         const int lane = q % 64, nb = q / 64;
         const int u = 32 * nb + lane % 32;
         bf16x8 a = {};
         if (lane < 32) {
-            const float cb = TAB_S * P[OB1 + u];
+            const float cb = TANH_C * P[OB1 + u];
             const __bf16 bh = (__bf16)cb;
 #pragma unroll
             for (int v = 0; v < 3; ++v) a[2 * v] = a[2 * v + 1] = (__bf16)P[OW1 + 3 * u + v];
@@ -355,28 +357,16 @@ __global__ __launch_bounds__(256) void pack_kernel(const float *__restrict__ P, 
     for (int q = t; q < 2 * 2 * 16; q += blockDim.x) {
         const int r = q % 16, h = (q / 16) % 2, nb = q / 32;
         const int u = 32 * nb + d_unit(h, r);
-        w->C2[nb][h][r] = TAB_S * P[OB2 + u] + TAB_OFF;
-        w->W3[nb][h][r] = bf16_round(P[OW3 + u]);
+        w->C2[nb][h][r] = TANH_C * P[OB2 + u];
+        w->W3n[nb][h][r] = -2.0f * bf16_round(P[OW3 + u]);
     }
     for (int q = t; q < 2 * 4 * 64; q += blockDim.x) {
         const int lane = q % 64, qq = (q / 64) % 4, nb = q / 256;
         const int row = 32 * nb + lane % 32, h = lane / 32;
         bf16x8 b;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) b[e] = (__bf16)(TAB_S * P[OW2 + row * NH + k_unit(qq, h, e)]);
+        for (int e = 0; e < 8; ++e) b[e] = (__bf16)(TANH_C * P[OW2 + row * NH + k_unit(qq, h, e)]);
         w->A2[nb][qq][lane] = b;
-    }
-    // the activation table: each interval's parabola through tanh at its
-    // ends and its midpoint, from FP64 tanh; c1 rounded to FP16 first and c2
-    // taken so that the parabola still ends at tanh(z_{m+1})
-    for (int m = t; m < TAB_N; m += blockDim.x) {
-        const double z0 = (double)m / TAB_S - TAB_Z;
-        const double y0 = tanh(z0), ym = tanh(z0 + 0.5 / TAB_S), y1 = tanh(z0 + 1.0 / TAB_S);
-        const float c0 = (float)y0;
-        const _Float16 c1 = (_Float16)(y1 - y0 - 2.0 * (y1 - 2.0 * ym + y0));
-        w->tab[m].c0 = c0;
-        w->tab[m].c1 = c1;
-        w->tab[m].c2 = (_Float16)(y1 - (double)c0 - (double)c1);
     }
 }
 

@@ -6,26 +6,9 @@
 // 64 cells of a wavefront at once, in the transposed form H^T = W X^T: hidden
 // units are the rows of the v_mfma_f32_32x32x16_bf16 products and cells their
 // columns.  The weights are then the A operands, the same in every call, and
-// the activations of one layer's result registers are the next layer's B
-// operand as they stand (the K order of the hidden layer follows the result
-// layout).  The VALU evaluates only the activations and the output dot
-// product.
-//
-// Activations: tanh by quadratic interpolation in a table of TAB_N intervals
-// over [-TAB_Z, TAB_Z] (each interval's parabola through its ends and its
-// midpoint, eight bytes per interval: within 1e-6 of tanh; |z| > TAB_Z takes
-// tanh(+-TAB_Z), within 2.3e-7 of +-1).  Linear interpolation (2.3e-5) is not enough: the output
-// layer sums 64 activations weighted by W3, and the interpolation error has
-// one sign per interval, so it adds up to hundredths of a kelvin.  The
-// pre-activations come out of the MFMAs already in table units,
-// f = TAB_S z + TAB_OFF: TAB_S = 64 is a power of two, so the scaled bf16
-// weights are the unscaled ones to the bit, and the offset is the
-// accumulator's start value.  One activation is then a clamp, a conversion,
-// a fraction, an address, one 8-byte LDS read of the interval's coefficients
-// and two FMAs (one of them mixed FP16/FP32): six VALU instructions against
-// two transcendentals (each four FMA issue slots) and two more for
-// 1 - 2 / (2^u + 1).  (A 16-byte entry made the table reads, 1 KB per
-// wavefront and activation, the kernel's limit.)
+// the tanh of one layer's result registers is the next layer's B operand as
+// it stands (the K order of the hidden layer follows the result layout).  The
+// VALU evaluates only the activations and the output dot product.
 #pragma once
 #include "common.h"
 
@@ -51,20 +34,9 @@ struct Norm {
     float mu_T, sd_T;
 };
 
-constexpr int TAB_N = 1024;
-constexpr float TAB_Z = 8.0f;
-constexpr float TAB_S = TAB_N / (2.0f * TAB_Z);  // 64, a power of two
-constexpr float TAB_OFF = TAB_N / 2;
-// the largest float below TAB_N: the clamp keeps the interval index < TAB_N
-constexpr float TAB_FMAX = 1023.99993896484375f;
-
-// One interval of the activation table: tanh(z_m + t / TAB_S) ~ c0 + t (c1 +
-// t c2), c0 in FP32, the small c1 (< 1/64) and c2 (< 1e-4) in FP16, c2
-// absorbing c1's rounding at t = 1: eight bytes, one LDS read per activation.
-struct alignas(8) TabEntry {
-    float c0;
-    _Float16 c1, c2;
-};
+// tanh(x) = 1 - 2 / (2^(TANH_C x) + 1): the pre-activations are formed
+// already scaled by TANH_C = 2 log2(e).
+constexpr float TANH_C = 2.8853900817779268f;
 
 // Result layout of the 32x32 product: D[i][j] (i = hidden unit, j = cell)
 // sits in lane j % 32 + 32 ((i / 4) % 2), register (i / 8) * 4 + i % 4.  So
@@ -78,51 +50,25 @@ __host__ __device__ constexpr int k_unit(int q, int h, int e) { return 32 * (q /
 // Packed inference block (lane = 64 lanes of a wavefront, h = lane / 32).
 //  A1[nb][lane]: for h = 0, the eight bf16 (W1[u][0], W1[u][0], W1[u][1],
 //     W1[u][1], W1[u][2], W1[u][2], bh, bl) of unit u = 32 nb + lane % 32, with
-//     bh + bl = TAB_S b1[u] split in two bf16; zero for h = 1.  The matching
-//     B operand holds (y0h, y0l, y1h, y1l, y2h, y2l, 1, 1), yh + yl = TAB_S x.
-//  C2[nb][h][r]: TAB_S b2[32 nb + d_unit(h, r)] + TAB_OFF, the hidden layer's
-//     start value.
-//  A2[nb][q][lane]: the eight bf16 of TAB_S W2[32 nb + lane % 32][k_unit(q, h, e)].
-//  W3[nb][h][r]: bf16(W3[32 nb + d_unit(h, r)]); y = b3 + sum_u W3[u] tanh(z_u).
-//  tab[m]: tanh(z_m + t / TAB_S) ~ c0 + t (c1 + t c2) on t in [0, 1],
-//     z_m = m / TAB_S - TAB_Z.
+//     bh + bl = TANH_C b1[u] split in two bf16; zero for h = 1.  The matching
+//     B operand holds (y0h, y0l, y1h, y1l, y2h, y2l, 1, 1), yh + yl = TANH_C x.
+//  C2[nb][h][r]: TANH_C b2[32 nb + d_unit(h, r)], the hidden layer's start value.
+//  A2[nb][q][lane]: the eight bf16 of TANH_C W2[32 nb + lane % 32][k_unit(q, h, e)].
+//  W3n[nb][h][r]: -2 bf16(W3[32 nb + d_unit(h, r)]); with b3s = b3 + sum of the
+//     bf16 W3, y = b3s + sum_u W3n[u] / (2^z_u + 1) = b3 + sum_u bf16(W3[u]) tanh.
 struct Weights {
     Norm nm;
-    float b3;
+    float b3s;
     alignas(16) bf16x8 A1[2][64];
     alignas(16) float C2[2][2][16];
-    alignas(16) float W3[2][2][16];
+    alignas(16) float W3n[2][2][16];
     alignas(16) bf16x8 A2[2][4][64];
-    alignas(16) TabEntry tab[TAB_N];
 };
 
-// tanh of the pre-activation f = TAB_S z + TAB_OFF (table units).
-__device__ __forceinline__ float tanh_tab(const TabEntry *tab, float f) {
-    f = __builtin_amdgcn_fmed3f(f, 0.0f, TAB_FMAX);
-    // one 8-byte LDS read (a struct copy would become two 4-byte reads)
-    const unsigned long long bits =
-        *reinterpret_cast<const unsigned long long *>(__builtin_assume_aligned(tab + (unsigned)f, 8));
-    const float c0 = __builtin_bit_cast(float, (unsigned)bits);
-    const _Float16 c1 = __builtin_bit_cast(_Float16, (unsigned short)(bits >> 32));
-    const _Float16 c2 = __builtin_bit_cast(_Float16, (unsigned short)(bits >> 48));
-    const float t = __builtin_amdgcn_fractf(f);
-    return __builtin_fmaf(t, __builtin_fmaf(t, (float)c2, (float)c1), c0);
-}
-
-#ifndef PYCLES_MLP_TAB
-#define PYCLES_MLP_TAB 2
-#endif
-// 1 / (2^u + 1); tanh(x) = 1 - 2 rcp_exp1(2 log2(e) x).
+// 1 / (2^u + 1); tanh(x) = 1 - 2 rcp_exp1(TANH_C x).  Saturates through
+// 2^u = inf or 0, with no clamp.
 __device__ __forceinline__ float rcp_exp1(float u) {
     return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
-}
-// tanh of layer L's pre-activation f (table units): by the table when the
-// build selects it for that layer, else by exp2 and rcp
-template <int L>
-__device__ __forceinline__ float act(const TabEntry *tab, float f) {
-    if (PYCLES_MLP_TAB >= L) return tanh_tab(tab, f);
-    constexpr float C = 2.8853900817779268f / TAB_S;  // 2 log2(e) per table unit
-    return __builtin_fmaf(-2.0f, rcp_exp1(__builtin_fmaf(f, C, -TAB_OFF * C)), 1.0f);
 }
 
 // Copy the packed block into LDS (all threads of the block).
@@ -139,9 +85,6 @@ __device__ __forceinline__ float guess_wave(const Weights *w, float x0, float x1
     const int h = lane >> 5;
     const int c = lane & 31;
     float out = 0.0f;
-    f32x16 off;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) off[r] = TAB_OFF;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         // the weights are read from LDS again for each half: kept in
@@ -149,8 +92,8 @@ __device__ __forceinline__ float guess_wave(const Weights *w, float x0, float x1
         asm volatile("" ::: "memory");
         // cells 32 m + c: their inputs, scaled and split, in the lanes h = 0
         const int src = 32 * m + c;
-        const float y[3] = {TAB_S * __shfl(x0, src, 64), TAB_S * __shfl(x1, src, 64),
-                            TAB_S * __shfl(x2, src, 64)};
+        const float y[3] = {TANH_C * __shfl(x0, src, 64), TANH_C * __shfl(x1, src, 64),
+                            TANH_C * __shfl(x2, src, 64)};
         bf16x8 B1;
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
@@ -164,9 +107,10 @@ __device__ __forceinline__ float guess_wave(const Weights *w, float x0, float x1
         bf16x8 B2[4];
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            const f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w->A1[nb][lane], B1, off, 0, 0, 0);
+            const f32x16 z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w->A1[nb][lane], B1, f32x16{}, 0, 0, 0);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) B2[2 * nb + r / 8][r % 8] = (__bf16)act<1>(w->tab, z[r]);
+            for (int r = 0; r < 16; ++r)
+                B2[2 * nb + r / 8][r % 8] = (__bf16)__builtin_fmaf(-2.0f, rcp_exp1(z[r]), 1.0f);
         }
         float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -183,20 +127,20 @@ __device__ __forceinline__ float guess_wave(const Weights *w, float x0, float x1
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) z = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w->A2[nb][q][lane], B2[q], z, 0, 0, 0);
-            const float4 *w3 = reinterpret_cast<const float4 *>(w->W3[nb][h]);
+            const float4 *w3 = reinterpret_cast<const float4 *>(w->W3n[nb][h]);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const float4 t = w3[v];
-                part[0] = __builtin_fmaf(t.x, act<2>(w->tab, z[4 * v]), part[0]);
-                part[1] = __builtin_fmaf(t.y, act<2>(w->tab, z[4 * v + 1]), part[1]);
-                part[2] = __builtin_fmaf(t.z, act<2>(w->tab, z[4 * v + 2]), part[2]);
-                part[3] = __builtin_fmaf(t.w, act<2>(w->tab, z[4 * v + 3]), part[3]);
+                part[0] = __builtin_fmaf(t.x, rcp_exp1(z[4 * v]), part[0]);
+                part[1] = __builtin_fmaf(t.y, rcp_exp1(z[4 * v + 1]), part[1]);
+                part[2] = __builtin_fmaf(t.z, rcp_exp1(z[4 * v + 2]), part[2]);
+                part[3] = __builtin_fmaf(t.w, rcp_exp1(z[4 * v + 3]), part[3]);
             }
         }
         // the two lane halves hold the two halves of each cell's sum
         const float half_sum = (part[0] + part[1]) + (part[2] + part[3]);
         const float sum = half_sum + __shfl_xor(half_sum, 32, 64);
-        if (h == m) out = w->b3 + sum;
+        if (h == m) out = w->b3s + sum;
     }
     return __builtin_fmaf(w->nm.sd_T, out, w->nm.mu_T);
 }

@@ -12,14 +12,18 @@ profiles, step counts and time steps of the five configurations):
 
 The FP32 comparisons gate T and qt at 1e-5 relative.  Where the FP64 run
 itself moves by more than that from a perturbation the FP32 form cannot avoid
-(its own storage rounding of the initial state, run in FP64), the gate is
-that floor: the FP32 form's error must stay within twice the error that the
-rounding of its initial state alone causes."""
+(its own storage: the FP64 run with its state rounded to FP32 storage after
+every step, FP64 arithmetic throughout), the gate is that floor: the FP32
+form's error must stay within twice the error that FP32 storage alone causes.
+The FP64 comparison against the oracle is gated at 1e-12, or at ten times
+how far the oracle moves over the same steps from a one-ulp perturbation of
+all its fields, where the projection's conditioning amplifies rounding
+differences (FFT factorisation, pivot reciprocals) beyond that."""
 import numpy as np
 import pytest
 import torch
 
-from tests.helpers import oracle_sim, rel_err
+from tests.helpers import floor_tolerance, oracle_sim, rel_err, ulp_perturbed
 
 pytestmark = pytest.mark.gpu
 
@@ -37,16 +41,31 @@ def test_c2_moist256_fp64_against_the_oracle(gpu):
     sim.Th.max_iter = 0
     sim.run(max_steps=c["steps"])
     torch.cuda.synchronize()
-    osim = oracle_sim(Gr, RS, Th, fields, True, dt=c["dt"])
-    osim.begin_run()
-    for _ in range(c["steps"]):
-        osim.step()
-    errs = {n: rel_err(sim.PV.get_device_layout(n), Gr.to_device_layout(osim.pv(n))) for n in osim.pv_names}
-    for n in ("temperature", "ql", "dynamic_pressure"):
-        errs[n] = rel_err(sim.DV.get_device_layout(n), Gr.to_device_layout(osim.dv[n]))
+    runs = {}
+    start_ulp = dict(fields)
+    for n in ("u", "v", "w", "s", "qt"):
+        start_ulp = ulp_perturbed(start_ulp, n)
+    for key, start in (("oracle", fields), ("oracle_ulp", start_ulp)):
+        o = oracle_sim(Gr, RS, Th, start, True, dt=c["dt"])
+        o.begin_run()
+        for _ in range(c["steps"]):
+            o.step()
+        runs[key] = o
+    osim = runs["oracle"]
+
+    def errs_of(get_pv, get_dv):
+        e = {n: rel_err(get_pv(n), Gr.to_device_layout(osim.pv(n))) for n in osim.pv_names}
+        for n in ("temperature", "ql", "dynamic_pressure"):
+            e[n] = rel_err(get_dv(n), Gr.to_device_layout(osim.dv[n]))
+        return e
+    errs = errs_of(sim.PV.get_device_layout, sim.DV.get_device_layout)
+    o2 = runs["oracle_ulp"]
+    floor = errs_of(lambda n: Gr.to_device_layout(o2.pv(n)), lambda n: Gr.to_device_layout(o2.dv[n]))
+    tol = floor_tolerance(floor)
     print("moist256 FP64, 10 steps of 2 s, against the oracle:", errs, flush=True)
+    print("oracle from a one-ulp perturbation of every field, against the oracle:", floor, flush=True)
     assert sim.TS.t == osim.t == 20.0
-    assert max(errs.values()) <= 1e-12, errs
+    assert all(errs[n] <= tol[n] for n in errs), (errs, tol)
     # the divergence left by the last projection, at the oracle's level
     gw = Gr.dims.gw
     sl = (slice(gw, -gw),) * 3
@@ -61,12 +80,13 @@ def test_c2_moist256_fp64_against_the_oracle(gpu):
 
 def _fp32_gate(es, steps):
     e = es["errors_vs_fp64"]
-    floor = es["noise_floor_fp64"]["fp64_from_fp32"]
+    floor = es["noise_floor_fp64"]["fp64_fp32_storage"]
     for key in ("after_1_steps", f"after_{steps}_steps"):
         for f in ("temperature", "qt"):
             err = e[key][f]["max_rel"]
             fl = floor[key][f]["max_rel"]
-            print(f"{key} {f}: FP32 + learned {err:.3e}, FP64 from the FP32-rounded state {fl:.3e}", flush=True)
+            print(f"{key} {f}: FP32 + learned {err:.3e}, FP64 rounded to FP32 storage every step {fl:.3e}",
+                  flush=True)
             assert err <= max(1e-5, 2.0 * fl), (key, f, err, fl)
 
 
@@ -91,5 +111,6 @@ def test_c5_moist1024_weno9_fp32_learned_against_fp64(gpu):
     assert (c["nx"], c["ny"], c["nz"], c["gw"], c["order"], c["dt"], c["steps"]) == \
         (1024, 1024, 192, 5, (9, 5), 0.5, 20)
     guess, _ = bench.train_guess(Th, 22, 40)
-    es = bench.error_study(nl, Gr, RS, Th, fields, guess, c["steps"], floors=True)
+    # one floor run: five FP64-sized states of this size would not fit beside the FP32 one
+    es = bench.error_study(nl, Gr, RS, Th, fields, guess, c["steps"], floors=True, floor_kinds=("storage",))
     _fp32_gate(es, c["steps"])

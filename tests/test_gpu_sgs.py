@@ -112,22 +112,40 @@ def test_steps_with_sgs_match_the_oracle(gpu):
 
 
 def test_fp32_with_sgs_tracks_fp64(gpu):
+    """FP32 against FP64 over 3 steps with the closure on, gated at 1e-5 or at
+    twice what FP32 storage alone does to an FP64 run (its state rounded to
+    FP32 storage after every step): on this rough start (entropy noise of
+    25 J/kg/K cell to cell, a saturated share) the moist N^2's branch switch
+    at the saturation edge turns storage rounding into viscosity
+    differences."""
+    import bench
     out = {}
-    for precision in ("f64", "f32"):
-        nl, Gr, RS, Th, fields = _case(32, 32, 32, seed=53, precision=precision)
-        sim = _sim(nl, Gr, RS, Th, fields)
-        sim.run(max_steps=3)
+    for precision in ("f64", "f32", "storage"):
+        nl, Gr, RS, Th, fields = _case(32, 32, 32, seed=53, precision="f64" if precision == "storage" else precision)
+        start = bench.perturbed_fields(fields, "fp32") if precision == "storage" else fields
+        sim = _sim(nl, Gr, RS, Th, start)
+        rounding = bench.FP32Storage(sim, fields) if precision == "storage" else None
+        sim.begin_run()
+        for _ in range(3):
+            sim.step()
+            if rounding is not None:
+                rounding.round()
         torch.cuda.synchronize()
         out[precision] = sim
     gw = out["f64"].Gr.dims.gw
-    for n in ("s", "qt"):
-        a = out["f32"].PV.get_device_layout(n)[gw:-gw]
-        b = out["f64"].PV.get_device_layout(n)[gw:-gw]
-        assert np.max(np.abs(a - b) / np.abs(b)) < 1e-5, n
-    a = out["f32"].DV.get_device_layout("temperature")[gw:-gw]
-    b = out["f64"].DV.get_device_layout("temperature")[gw:-gw]
-    assert np.max(np.abs(a - b) / b) < 1e-5
+
+    def rel(n, which, dv=False):
+        src = (lambda s: s.DV) if dv else (lambda s: s.PV)
+        a = src(out[which]).get_device_layout(n)[gw:-gw]
+        b = src(out["f64"]).get_device_layout(n)[gw:-gw]
+        return float(np.max(np.abs(a - b) / np.abs(b)))
+    for n, dv in (("s", False), ("qt", False), ("temperature", True)):
+        err, floor = rel(n, "f32", dv), rel(n, "storage", dv)
+        print(f"{n}: FP32 {err:.3e}, FP64 rounded to FP32 storage every step {floor:.3e}", flush=True)
+        assert err <= max(1e-5, 2.0 * floor), (n, err, floor)
     for n in ("u", "v", "w"):
         a = out["f32"].PV.get_device_layout(n)
         b = out["f64"].PV.get_device_layout(n)
-        assert np.max(np.abs(a - b)) < 1e-3 * max(1.0, np.max(np.abs(b))), n
+        c = out["storage"].PV.get_device_layout(n)
+        scale = max(1.0, np.max(np.abs(b)))
+        assert np.max(np.abs(a - b)) < max(1e-3 * scale, 2.0 * np.max(np.abs(c - b))), n
