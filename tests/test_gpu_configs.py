@@ -104,9 +104,9 @@ def test_c3_c4_moist512_fp32_learned_against_fp64(gpu):
     assert info["heldout_rms_K"] < 0.05
     es = bench.error_study(nl, Gr, RS, Th, fields, guess, c["steps"], floors=True)
     # After 100 steps qt misses 1e-5: FP32 storage alone (an FP64 run rounded
-    # to it every step) gives 1.9e-5, and the FP32 form's arithmetic adds
-    # velocity errors of ~3x that run's per step (FP32 buoyancy and
-    # projection), which the qt fronts of the cloud layer turn into 2.3e-4
+    # to it every step) gives 1.9e-5, and the FP32 arithmetic injects as much
+    # velocity error again (the thermodynamics first: with it in FP64 the
+    # error halves), which the qt fronts of the cloud layer turn into 2.3e-4
     # (12x the floor; T stays at 5e-6).  The long-run qt bound here is that
     # measured level with a margin, a regression guard rather than the gate.
     _fp32_gate(es, c["steps"], long_qt_factor=20.0)
