@@ -50,14 +50,17 @@ METRIC = "Mcell-updates/s per RK3 step: FP32+learned vs plain FP64 at <=1e-5 rel
 #           their tendencies, writes the five new values (stage 0: 15 x 4);
 #           stages 1, 2 also read the stage copies (20 x 4); the launch time
 #           is the mean over the three stages, so are the bytes   (15 + 20 + 20) / 3 x 4
-KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, "adv_z_rk3": 220 / 3}
+#   adv_yz, adv_yz_rk3  the y and z passes as one kernel (FP32, WENO5): the
+#           bytes of the z pass alone (the y pass's own 15 x 4 disappear)
+KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, "adv_z_rk3": 220 / 3,
+                "adv_yz": 60, "adv_yz_rk3": 220 / 3}
 # Algorithmic flops per interior cell of the advection passes: WENO5 scalar and
 # momentum advection is ~2265 flop per cell per stage over the three directions
 # (2 scalar and 9 momentum face fluxes, interpolation and divergence), split
 # evenly over the directional passes; the fused RK3 stage adds 4 flop per
 # variable.  VALU peak: FP32 vector with packed FMA (the kernels issue no
 # packed FP32, so half of it is their ceiling).
-KERNEL_FLOPS = {"adv_x": 755, "adv_y": 755, "adv_z": 755, "adv_z_rk3": 775}
+KERNEL_FLOPS = {"adv_x": 755, "adv_y": 755, "adv_z": 755, "adv_z_rk3": 775, "adv_yz": 1510, "adv_yz_rk3": 1530}
 VALU_PEAK_TFLOPS = 157.3
 # Field passes per moist SSP-RK3 step (thermo 6, scalar advection 7,
 # momentum advection 7, RK update 20, projection 22 per stage; x3)
@@ -68,7 +71,8 @@ def kernel_names(so, mo):
     """Kernel instantiations timed as each phase (scalar order so, momentum order mo)."""
     adv = lambda d: f"adv::adv_{d}<PolicyF32,{so},{mo},2,true>"  # noqa: E731
     return {"thermo": "thermo_sa_f32_kernel<true,true>", "adv_x": adv("x"), "adv_y": adv("y"), "adv_z": adv("z"),
-            "rk3": "rk3_kernel<float,...>", "adv_z_rk3": adv("z") + " (RK3 fused)"}
+            "rk3": "rk3_kernel<float,...>", "adv_z_rk3": adv("z") + " (RK3 fused)", "adv_yz": adv("yz"),
+            "adv_yz_rk3": adv("yz") + " (RK3 fused)"}
 
 
 def log(msg):

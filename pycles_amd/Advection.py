@@ -75,6 +75,13 @@ class MomentumAdvection:
                         PV.precision, D.stream(), _capi.ADV_TEND_ZERO if tend_zero else 0)
 
 
+def yz_fused(SA: ScalarAdvection, MA: MomentumAdvection, PV) -> bool:
+    """Whether one call's y and z passes run as one fused kernel (the
+    library does so for FP32 storage, WENO5 scalars and momentum, with at
+    least one scalar); otherwise they are two kernels."""
+    return PV.precision == D.F32 and SA.order == 5 and MA.order == 5 and bool(_scalar_names(PV))
+
+
 def advect_rk3(SA: ScalarAdvection, MA: MomentumAdvection, TS, Gr, RS, PV, tend_zero=False, passes=0,
                arrays=None):
     """advect_fused followed by TS.update in one sweep (pycles_advection_rk3).

@@ -19,7 +19,7 @@ from __future__ import annotations
 import time
 
 from . import _device as D
-from .Advection import MomentumAdvection, ScalarAdvection, advect_fused, advect_rk3
+from .Advection import MomentumAdvection, ScalarAdvection, advect_fused, advect_rk3, yz_fused
 from .Grid import Grid
 from .ParallelMPI import NetCDFIO_Stats, ParallelMPI
 from .PressureSolver import PressureSolver
@@ -167,8 +167,11 @@ class Simulation3d:
         if fused_rk:
             if self.timers is not None:
                 from ._device import _capi
-                passes = (("adv_x", _capi.ADV_PASS_X), ("adv_y", _capi.ADV_PASS_Y),
-                          ("adv_z_rk3", _capi.ADV_PASS_Z))
+                if yz_fused(self.SA, self.MA, self.PV):
+                    passes = (("adv_x", _capi.ADV_PASS_X), ("adv_yz_rk3", _capi.ADV_PASS_Y | _capi.ADV_PASS_Z))
+                else:
+                    passes = (("adv_x", _capi.ADV_PASS_X), ("adv_y", _capi.ADV_PASS_Y),
+                              ("adv_z_rk3", _capi.ADV_PASS_Z))
             else:
                 passes = (("advection_rk3", 0),)
             arrays = self.PV.rk_arrays(rk_step)
@@ -179,11 +182,14 @@ class Simulation3d:
             self.PV.set_values(arrays[0])
             self.TS.advance()
         elif self.fuse_advection and self.timers is not None:
-            # the same three kernels as the fused call, with an event between
-            # each so every directional pass is timed on its own
+            # the same kernels as the fused call, with an event between each
+            # so every kernel is timed on its own
             from ._device import _capi
-            for name, bit in (("adv_x", _capi.ADV_PASS_X), ("adv_y", _capi.ADV_PASS_Y),
-                              ("adv_z", _capi.ADV_PASS_Z)):
+            if yz_fused(self.SA, self.MA, self.PV):
+                passes = (("adv_x", _capi.ADV_PASS_X), ("adv_yz", _capi.ADV_PASS_Y | _capi.ADV_PASS_Z))
+            else:
+                passes = (("adv_x", _capi.ADV_PASS_X), ("adv_y", _capi.ADV_PASS_Y), ("adv_z", _capi.ADV_PASS_Z))
+            for name, bit in passes:
                 self._mark(name)
                 advect_fused(self.SA, self.MA, self.Gr, self.Ref, self.PV, tend_zero=fast, passes=bit)
         elif self.fuse_advection:

@@ -26,6 +26,14 @@
 
 #include "common.h"
 
+// rows per block of the fused y-z pass
+#ifndef PYCLES_YZ_ROWS
+#define PYCLES_YZ_ROWS 4
+#endif
+#ifndef PYCLES_YZ_WAVES
+#define PYCLES_YZ_WAVES 3
+#endif
+
 namespace adv {
 
 // Calls f(integral_constant<int, Q>) for Q = Q0 .. N-1 while it returns true;
@@ -95,6 +103,10 @@ struct PolicyF64 {
     using TC = double;
     using SFlux = double;
     static constexpr bool kBase = false;
+    // the y and z passes stay separate: the fused kernel's double-buffered
+    // FP64 tiles (86 KB of LDS) leave one block per CU
+    static constexpr bool kFuseYZ = false;
+    static constexpr int kYZWaves = 1;
     template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *, TC U, double rho) {
         return upwind_flux<O, double>(win, U, rho);
@@ -141,6 +153,8 @@ struct PolicyF32 {
     using TC = float;
     using SFlux = SF32;
     static constexpr bool kBase = true;
+    static constexpr bool kFuseYZ = true;  // y and z passes fused (WENO5)
+    static constexpr int kYZWaves = PYCLES_YZ_WAVES;     // waves per SIMD of the fused pass
     template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *bwin, TC U, double rho) {
         constexpr int R = Interp<O, float>::R;
@@ -698,6 +712,340 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
 }
 
 // -------------------------------------------------------------------------
+// y and z passes fused (WENO5 scalars and momentum): one thread per (i, j)
+// column marching the faces c = k0-1 .. k1-1 of a z chunk as the z pass does,
+// with the y face fluxes of each level computed from LDS.  A block of 64 x TY
+// threads covers 64 columns of TY rows.  Per face c:
+//  A. stage level c of every field in LDS for the block's rows (from the
+//     march's windows) and R halo rows on each side, and v and w with the x
+//     halo of the centred stencils (halo values loaded one face ahead); then
+//     issue this cell's tendency and stage-copy loads, the next window level
+//     and the next face's halo.
+//  B. (barrier) the z face flux from the windows, with w centred along x and
+//     y from the tile; the y flux of the face above each row from the tile,
+//     into LDS; the block's lowest face (below row j0) one variable per wave.
+//  C. (barrier) each cell's tendency takes the y flux difference (the lower
+//     face from LDS), rounded as the y pass stores it, then the z flux
+//     difference, in the per-variable order of the separate passes, so the
+//     bits are theirs; then the RK3 stage as in the z pass.
+// Against the y pass followed by the z pass this drops the y pass's tendency
+// read and write and its re-read of the fields.  Columns beyond a partial
+// tile (nx not a multiple of 64, ny of TY; PART) march wrapped copies of
+// real columns and store nothing, so every thread reaches every barrier.
+// A chunk holds at most YZ_KMAX levels: its level profiles are staged in LDS
+// (read from global memory they are vector loads, whose waits would drain the
+// prefetches in flight).
+constexpr int YZ_KMAX = 128;
+
+template <class P, int SO, int MO, int NSC, bool MOM, int RKM, int TY, bool PART>
+__global__ __launch_bounds__(64 * TY, P::kYZWaves) void adv_yz(Geo g, Prof pr, Fields fl, int kchunk) {
+    using TS = typename P::TS;
+    using TC = typename P::TC;
+    using SFlux = typename P::SFlux;
+    constexpr int R = Interp<SO, TC>::R;
+    static_assert(SO == 5 && MO == 5 && MOM && NSC >= 1 && NSC <= 2, "fused y-z pass: WENO5 scalars and momentum");
+    constexpr int NC = 2 * (R - 1);  // centred stencil (interp_4)
+    constexpr int NR = NC + 1;       // ring of v levels: NC read per face, one staged ahead
+    constexpr int WN = 2 * R;        // WENO window
+    constexpr int XL = R - 2;        // centred x stencil: i - XL .. i + XL + NC - 1
+    constexpr int TX = 64 + NC - 1;
+    constexpr int TR = TY + 2 * R;  // tile rows j0 - R .. j0 + TY + R - 1
+    constexpr int NV = NSC + 3;
+    constexpr int FU = NSC, FV = NSC + 1, FW = NSC + 2;  // tile field index
+    constexpr int NHR = (2 * R + TY - 1) / TY;           // halo rows per thread row
+    constexpr int NXH = (2 * TY + 1) * (NC - 1);         // x halo items (v: TY + 1 rows, w: TY rows)
+    constexpr int NPL = YZ_KMAX + 2 * R;                 // profile levels k0 - R .. k0 + kchunk + R - 1
+    constexpr int NPR = P::kBase ? 4 + NSC : 4;
+    static_assert(NXH <= 64 * TY, "x halo items exceed the block");
+    // tile and face fluxes double-buffered by level parity: level c + 1 is
+    // staged while slower waves still read level c, so one barrier per face
+    __shared__ TC tile[2][NV][TR][TX];
+    __shared__ TC flx[2][NV][TY + 1][64];  // y flux of face j0 - 1 + f + 1/2 in slot f
+    __shared__ TC vring[NR][64];           // v of row j0 - 1 at level lev in slot lev % NR
+    __shared__ double lp[NPR][NPL];        // rho0, rho0_half, alpha0, alpha0_half (, scalar bases)
+    const uint3 tl = xcd_tile<1>();        // XCD bands of TY-row blocks in y
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int nx = g.nx, ny = g.ny;
+    const int ib = tl.x * 64, j0 = tl.y * TY;
+    // a real column, which stores its cells (PART: the grid has partial
+    // tiles; otherwise every thread owns its column, and the stores are not
+    // under a branch, into which the compiler would sink the tendency loads)
+    const bool own = !PART || (ib + tx < nx && j0 + ty < ny);
+    const int iw = (ib + tx) % nx, jw = (j0 + ty) % ny;
+    const int k0 = g.gw + tl.z * kchunk;
+    const int k1 = min(k0 + kchunk, g.nz + g.gw);
+    const int ktop = g.nzg - 1;
+    const int kb = k0 - R;  // level of profile slot 0
+    const unsigned plane = (unsigned)g.plane;
+    const unsigned col = (unsigned)jw * nx + iw;
+    auto wrap = [](int a, int n) { return ((a % n) + n) % n; };
+    for (int l = ty * 64 + tx; l < NPL; l += 64 * TY) {
+        const int lev = min(kb + l, ktop);
+        lp[0][l] = pr.rho0[lev];
+        lp[1][l] = pr.rho0_half[lev];
+        lp[2][l] = pr.alpha0[lev];
+        lp[3][l] = pr.alpha0_half[lev];
+        if (P::kBase) {
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) lp[(4 + sc) % NPR][l] = fl.base[sc] ? fl.base[sc][lev] : 0.0;
+        }
+    }
+    auto prof = [&](int q, int lev) { return lp[q][lev - kb]; };
+    const TS *pf[NV];
+#pragma unroll
+    for (int sc = 0; sc < NSC; ++sc) pf[sc] = (const TS *)fl.q[sc];
+    pf[FU] = (const TS *)fl.q[VU];
+    pf[FV] = (const TS *)fl.q[VV];
+    pf[FW] = (const TS *)fl.q[VW];
+    // halo rows of this thread row: hr = ty + h TY < 2R, tile row hr (below)
+    // or TY + hr (above); the others load row 0's cells and store nothing
+    unsigned hoff[NHR];
+    int hrow[NHR];
+    bool hval[NHR];
+#pragma unroll
+    for (int h = 0; h < NHR; ++h) {
+        const int hr = ty + h * TY;
+        hval[h] = hr < 2 * R;
+        hrow[h] = !hval[h] ? 0 : (hr < R ? hr : TY + hr);
+        hoff[h] = (unsigned)wrap(j0 - R + hrow[h], ny) * nx + iw;
+    }
+    // x halo item of this thread: v of tile rows R-1 .. R+TY-1, then w of
+    // tile rows R .. R+TY-1; tile columns 0 .. XL-1 and 64+XL .. TX-1
+    const int tl1 = ty * 64 + tx;
+    const bool xval = tl1 < NXH;
+    const int rf = (xval ? tl1 : 0) / (NC - 1), cs = (xval ? tl1 : 0) % (NC - 1);
+    const bool xw = rf > TY;
+    const int xrow = xw ? R + (rf - TY - 1) : R - 1 + rf;
+    const int xcol = cs < XL ? cs : 64 + cs;
+    const unsigned xoff = (unsigned)wrap(j0 - R + xrow, ny) * nx + wrap(ib - XL + xcol, nx);
+    const TS *xptr = xw ? pf[FW] : pf[FV];
+    const unsigned voff = (unsigned)wrap(j0 - 1, ny) * nx + iw;  // v of row j0 - 1
+
+    // windows of face c: level c - (R-1) + m in slot m
+    TC rs[NSC][WN], ru[WN], rv[WN], rw[WN];
+    int c = k0 - 1;
+#pragma unroll
+    for (int m = 0; m < WN; ++m) {
+        const unsigned o = (unsigned)(c - (R - 1) + m) * plane + col;
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) rs[sc][m] = (TC)ld(pf[sc], o);
+        ru[m] = (TC)ld(pf[FU], o);
+        rv[m] = (TC)ld(pf[FV], o);
+        rw[m] = (TC)ld(pf[FW], o);
+    }
+    // the halo of the tile at level lev and the ring's level lev + 2, for
+    // staging level lev (every load unconditional; the ring and x halo first)
+    TC hp[NHR][NV], xh, vr;
+    auto prefetch = [&](int lev) {
+        const unsigned lo = (unsigned)min(lev, ktop) * plane;
+        vr = (TC)ld(pf[FV], (unsigned)min(lev + 2, ktop) * plane + voff);
+        xh = (TC)ld(xptr, lo + xoff);
+#pragma unroll
+        for (int h = 0; h < NHR; ++h)
+#pragma unroll
+            for (int v = 0; v < NV; ++v) hp[h][v] = (TC)ld(pf[v], lo + hoff[h]);
+    };
+    // stage level cc into tile[cc & 1]; slot m of the windows holds it
+    auto stage = [&](int cc, int m) {
+        auto &t = tile[cc & 1];
+        if (ty == TY - 1) vring[(cc + 2) % NR][tx] = vr;
+        if (xval) t[xw ? FW : FV][xrow][xcol] = xh;
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) t[sc][R + ty][XL + tx] = rs[sc][m];
+        t[FU][R + ty][XL + tx] = ru[m];
+        t[FV][R + ty][XL + tx] = rv[m];
+        t[FW][R + ty][XL + tx] = rw[m];
+#pragma unroll
+        for (int h = 0; h < NHR; ++h)
+            if (hval[h]) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) t[v][hrow[h]][XL + tx] = hp[h][v];
+            }
+    };
+    prefetch(c);
+    if (ty == TY - 1) {
+#pragma unroll
+        for (int m = 0; m < NC - 1; ++m) {
+            const int lev = c - 1 + m;
+            vring[lev % NR][tx] = (TC)ld(pf[FV], (unsigned)lev * plane + voff);
+        }
+    }
+    stage(c, R - 1);
+    prefetch(c + 1);
+    __syncthreads();
+
+    SFlux Fs[NSC], Fs_prev[NSC];
+    TC Fu = 0, Fv = 0, Fw = 0, Fu_prev = 0, Fv_prev = 0, Fw_prev = 0;
+    const double dzi = g.dxi[2], dyi = g.dxi[1];
+    // one face; FIRST: the chunk's lowest face (k0 - 1), z flux only.
+    //  B. fluxes from tile[c & 1]; y fluxes into flx[c & 1]; this cell's
+    //     tendency loads; level c + 1 staged into tile[(c + 1) & 1]; the
+    //     next window level and the halo of level c + 2 loaded
+    //  (barrier)
+    //  C. flux differences, stage update, window shift
+    auto face = [&](auto firstc) {
+        constexpr bool FIRST = decltype(firstc)::value;
+        const unsigned o = (unsigned)c * plane + col;
+        const int b = c & 1;
+        // the z face flux
+        {
+            TC cwx[NC], cwy[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) {
+                cwx[m] = tile[b][FW][R + ty][tx + m];            // w along x
+                cwy[m] = tile[b][FW][R + ty - XL + m][XL + tx];  // w along y
+            }
+            const double rho0c = prof(0, c);
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) {
+                double bz[WN];
+#pragma unroll
+                for (int m = 0; m < WN; ++m) bz[m] = P::kBase ? prof((4 + sc) % NPR, c - (R - 1) + m) : 0.0;
+                Fs[sc] = P::template scal_flux<SO, true>(rs[sc], bz, rw[R - 1], rho0c);
+            }
+            Fu = P::template mom_flux<MO>(ru, Interp<MO, TC>::centred(cwx), rho0c);
+            Fv = P::template mom_flux<MO>(rv, Interp<MO, TC>::centred(cwy), rho0c);
+            TC q[NC];
+#pragma unroll
+            for (int m = 0; m < NC; ++m) q[m] = rw[m + 1];  // w along z
+            Fw = P::template mom_flux<MO>(rw, Interp<MO, TC>::centred(q), prof(1, c + 1));
+        }
+        TC Fy[NV] = {}, Vp = 0;
+        const TC Vc = rv[R - 1];
+        const double rh = prof(1, c), r0 = prof(0, c);
+        TS told[5] = {}, cpv[5] = {};
+        if constexpr (!FIRST) {
+            // the face above this row: window rows j-R+1 .. j+R = tile rows ty+1 ..
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                TC win[WN];
+#pragma unroll
+                for (int m = 0; m < WN; ++m) win[m] = tile[b][v][ty + 1 + m][XL + tx];
+                TC q[NC];
+                if (v < NSC) {
+                    const double bdum[WN] = {};
+                    Fy[v] = P::x_keep(P::template scal_flux<SO, false>(win, bdum, Vc, rh));
+                } else if (v == FU) {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) q[m] = tile[b][FV][R + ty][tx + m];  // v along x
+                    Fy[v] = P::template mom_flux<MO>(win, Interp<MO, TC>::centred(q), rh);
+                } else if (v == FV) {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) q[m] = win[m + 1];  // v along y
+                    Fy[v] = P::template mom_flux<MO>(win, Interp<MO, TC>::centred(q), rh);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) q[m] = rv[m + 1];  // v along z
+                    Fy[v] = P::template mom_flux<MO>(win, Interp<MO, TC>::centred(q), r0);
+                }
+                flx[b][v][ty + 1][tx] = Fy[v];
+            }
+            Vp = tile[b][FV][R + ty - 1][XL + tx];
+            // the face below row j0, one variable per wave (uniform branches)
+            for (int ev = ty; ev < NV; ev += TY) {
+                TC win[WN], q[NC];
+#pragma unroll
+                for (int m = 0; m < WN; ++m) win[m] = tile[b][ev][m][XL + tx];
+                TC F;
+                if (ev < NSC) {
+                    const double bdum[WN] = {};
+                    F = P::x_keep(P::template scal_flux<SO, false>(win, bdum, tile[b][FV][R - 1][XL + tx], rh));
+                } else if (ev == FU) {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) q[m] = tile[b][FV][R - 1][tx + m];
+                    F = P::template mom_flux<MO>(win, Interp<MO, TC>::centred(q), rh);
+                } else if (ev == FV) {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) q[m] = win[m + 1];
+                    F = P::template mom_flux<MO>(win, Interp<MO, TC>::centred(q), rh);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < NC; ++m) q[m] = vring[(c - 1 + m) % NR][tx];
+                    F = P::template mom_flux<MO>(win, Interp<MO, TC>::centred(q), r0);
+                }
+                flx[b][ev][0][tx] = F;
+            }
+            // this cell's tendencies and stage copies, read before any store
+            // (the outputs may alias the copies), after the flux arithmetic,
+            // which keeps them out of its register peak
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                if (v < NSC || v >= VU) {
+                    told[v] = lds_((const TS *)fl.t[v], o);
+                    if (RKM == 2) cpv[v] = lds_((const TS *)fl.cp[v], o);
+                }
+        }
+        // level c + 1 (window slot R) into the other tile, then the loads for
+        // the next face
+        stage(c + 1, R);
+        TC nws[NSC], nu, nv, nw;
+        {
+            const unsigned on = (unsigned)min(c + 1 + R, ktop) * plane + col;
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) nws[sc] = (TC)ld(pf[sc], on);
+            nu = (TC)ld(pf[FU], on);
+            nv = (TC)ld(pf[FV], on);
+            nw = (TC)ld(pf[FW], on);
+        }
+        prefetch(c + 2);
+        __syncthreads();
+        if constexpr (!FIRST) {
+            // C: y then z flux differences, then the stage update
+            const double ah = prof(3, c), a0 = prof(2, c);
+            TS tn[5] = {};
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) {
+                const double brho = P::kBase ? prof((4 + sc) % NPR, c) * rh : 0.0;
+                const TS t1 = told[sc] - P::x_scal_div(Fy[sc], flx[b][sc][ty][tx], Vc, Vp, ah, dyi, brho);
+                tn[sc] = add_rn(t1, -P::template scal_div<true>(Fs[sc], Fs_prev[sc], ah, dzi, 1.0));
+            }
+            const TS tu = (TS)(told[VU] - P::mom_div(Fy[FU], flx[b][FU][ty][tx], ah, dyi));
+            const TS tv = (TS)(told[VV] - P::mom_div(Fy[FV], flx[b][FV][ty][tx], ah, dyi));
+            const TS tw = (TS)(told[VW] - P::mom_div(Fy[FW], flx[b][FW][ty][tx], a0, dyi));
+            tn[VU] = add_rn(tu, -P::mom_div(Fu, Fu_prev, ah, dzi));
+            tn[VV] = add_rn(tv, -P::mom_div(Fv, Fv_prev, ah, dzi));
+            tn[VW] = add_rn(tw, -P::mom_div(Fw, Fw_prev, a0, dzi));
+            if (own) {
+                if (RKM == 0) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        if (v < NSC || v >= VU) sts_((TS *)fl.t[v], o, tn[v]);
+                } else {
+#pragma unroll
+                    for (int sc = 0; sc < NSC; ++sc)
+                        rk_store<TS, RKM>(fl, sc, o, col, plane, c, g.gw, g.nzg, (TS)rs[sc][R - 1], tn[sc], cpv[sc],
+                                          false);
+                    rk_store<TS, RKM>(fl, VU, o, col, plane, c, g.gw, g.nzg, (TS)ru[R - 1], tn[VU], cpv[VU], false);
+                    rk_store<TS, RKM>(fl, VV, o, col, plane, c, g.gw, g.nzg, (TS)rv[R - 1], tn[VV], cpv[VV], false);
+                    rk_store<TS, RKM>(fl, VW, o, col, plane, c, g.gw, g.nzg, (TS)rw[R - 1], tn[VW], cpv[VW], true);
+                }
+            }
+        }
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) Fs_prev[sc] = Fs[sc];
+        Fu_prev = Fu;
+        Fv_prev = Fv;
+        Fw_prev = Fw;
+        // shift the windows
+#pragma unroll
+        for (int m = 0; m < WN - 1; ++m) {
+#pragma unroll
+            for (int sc = 0; sc < NSC; ++sc) rs[sc][m] = rs[sc][m + 1];
+            ru[m] = ru[m + 1];
+            rv[m] = rv[m + 1];
+            rw[m] = rw[m + 1];
+        }
+#pragma unroll
+        for (int sc = 0; sc < NSC; ++sc) rs[sc][WN - 1] = nws[sc];
+        ru[WN - 1] = nu;
+        rv[WN - 1] = nv;
+        rw[WN - 1] = nw;
+    };
+    face(std::true_type{});
+    for (c = k0; c < k1; ++c) face(std::false_type{});
+}
+
+// -------------------------------------------------------------------------
 // x pass: one block per (j, k) row, in three phases separated by barriers.
 //  1. stage in LDS: the rows of every field (plus a periodic halo of H
 //     cells), the advecting u centred along y and z for the v and w fluxes
@@ -903,6 +1251,30 @@ int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
     if (fl.passes & 1) {
         hipLaunchKernelGGL((adv_x<P, SO, MO, NSC, MOM>), dim3(g.ny, g.nz), dim3(bx), smem, st, g, pr, fl);
         PYCLES_CHECK_LAUNCH();
+    }
+    // y and z fused (WENO5 with momentum and scalars), z-chunked like the z pass
+    if constexpr (P::kFuseYZ && SO == 5 && MO == 5 && MOM && NSC >= 1) {
+        if ((fl.passes & 6) == 6) {
+            constexpr int TY = PYCLES_YZ_ROWS;
+            ZSplit zs = advz_zsplit(g.nx, g.ny, g.nz);
+            if (zs.len > YZ_KMAX) zs = zsplit_ceil(g.nz, (g.nz + YZ_KMAX - 1) / YZ_KMAX);
+            const dim3 grid((g.nx + 63) / 64, (g.ny + TY - 1) / TY, zs.nch), blk(64, TY);
+            const bool part = g.nx % 64 != 0 || g.ny % TY != 0;
+            const int rkm = fl.rk < 0 ? 0 : (fl.rk == 0 ? 1 : 2);
+#define ADV_YZ(RKM_, PART_) hipLaunchKernelGGL((adv_yz<P, SO, MO, NSC, MOM, RKM_, TY, PART_>), grid, blk, 0, st, g, pr, fl, zs.len)
+            if (part) {
+                if (rkm == 0) ADV_YZ(0, true);
+                else if (rkm == 1) ADV_YZ(1, true);
+                else ADV_YZ(2, true);
+            } else {
+                if (rkm == 0) ADV_YZ(0, false);
+                else if (rkm == 1) ADV_YZ(1, false);
+                else ADV_YZ(2, false);
+            }
+#undef ADV_YZ
+            PYCLES_CHECK_LAUNCH();
+            return PYCLES_OK;
+        }
     }
     // y: split the march into chunks so the grid holds enough waves.
     const ZSplit ys = advy_zsplit(g.nx, g.ny, g.nz);

@@ -70,6 +70,65 @@ def test_advection_separate_equals_fused(gpu):
     assert torch.equal(fused, sim.PV.tendencies)
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(24, 20, 16), (64, 8, 97), (96, 13, 33)])
+def test_fused_yz_matches_separate_passes(gpu, precision, shape):
+    """One advection call runs the x pass and then the fused y-z kernel
+    (WENO5); the x, y and z passes launched one at a time give the same bits
+    in FP64, for the tendencies and for the RK3 stage fused into the z march
+    (every stage, ghost levels included).  The FP32 kernels are compiled with
+    multiply-adds fused, which the compiler forms per kernel, so there the two
+    agree to rounding: within 1e-6 of each field's largest tendency (~8 ulps;
+    measured up to 1.7e-7) and of its largest value plus dt times that.  The
+    shapes give partial 64-column and 4-row tiles and z chunks of uneven
+    length."""
+    import json
+    from pycles_amd import _capi
+    from pycles_amd.Advection import advect_fused, advect_rk3
+    nl, Gr, RS, Th, fields = host_case(*shape, seed=21)
+    nl = json.loads(json.dumps(nl))
+    nl["mi355x"]["precision"] = precision
+    sim = _gpu_sim(nl, Gr, RS, Th, _swirl(Gr, fields))
+    PV = sim.PV
+    X, Y, Z = _capi.ADV_PASS_X, _capi.ADV_PASS_Y, _capi.ADV_PASS_Z
+    print(f"fused y-z: {precision} {shape}", flush=True)
+
+    def tendencies(seq):
+        PV.zero_tendencies()
+        for p in seq:
+            advect_fused(sim.SA, sim.MA, sim.Gr, sim.Ref, PV, passes=p)
+        torch.cuda.synchronize()
+        return PV.tendencies.clone()
+
+    def agree(a, b, scale, what):
+        if precision == "f64":
+            assert torch.equal(a, b), what
+            return
+        for v, n in enumerate(PV.index_name):
+            d = (a[v].double() - b[v].double()).abs().max().item()
+            assert d <= 1e-6 * scale[v], (what, n, d, scale[v])
+
+    fused, sep = tendencies([0]), tendencies([X, Y, Z])
+    tscale = [sep[v].double().abs().max().item() for v in range(len(PV.index_name))]
+    agree(fused, sep, tscale, "tendencies")
+    cp = PV.values.clone()
+    cp.mul_(1.0 + 1e-3)
+    sim.TS.dt = 0.5
+    for rk in range(3):
+        sim.TS.rk_step = rk
+        outs = []
+        for seq in ([0], [X, Y, Z]):
+            PV.zero_tendencies()
+            out = torch.zeros_like(PV.values)
+            for p in seq:
+                advect_rk3(sim.SA, sim.MA, sim.TS, sim.Gr, sim.Ref, PV, passes=p,
+                           arrays=(out, None if rk == 0 else cp))
+            torch.cuda.synchronize()
+            outs.append(out)
+        vscale = [outs[1][v].double().abs().max().item() + 0.5 * tscale[v] for v in range(len(tscale))]
+        agree(outs[0], outs[1], vscale, f"stage {rk}")
+
+
 def _saturating(Gr, fields, seed=3):
     """Cool a band of cells so a good share of them saturates."""
     d = Gr.dims
