@@ -62,6 +62,7 @@ KERNEL_BYTES = {"thermo": 44, "adv_x": 44, "adv_y": 60, "adv_z": 60, "rk3": 80, 
 # packed FP32, so half of it is their ceiling).
 KERNEL_FLOPS = {"adv_x": 755, "adv_y": 755, "adv_z": 755, "adv_z_rk3": 775, "adv_yz": 1510, "adv_yz_rk3": 1530}
 VALU_PEAK_TFLOPS = 157.3
+CUS, CLOCK_GHZ = 256, 2.4
 # Field passes per moist SSP-RK3 step (thermo 6, scalar advection 7,
 # momentum advection 7, RK update 20, projection 22 per stage; x3)
 STEP_PASSES = 186
@@ -352,7 +353,8 @@ def cpu_baseline(c, Gr, RS, Th, fields):
 
 
 def measured_traffic():
-    """HBM bytes per launch of each kernel from committed rocprofv3 --pmc runs."""
+    """HBM bytes and VALU wavefront instructions per launch of each kernel,
+    from committed rocprofv3 --pmc runs."""
     p = os.path.join(ROOT, "profiles", "traffic_moist512.json")
     if os.path.exists(p):
         with open(p) as f:
@@ -511,15 +513,22 @@ def main():
             v["flop_per_cell"] = KERNEL_FLOPS[k]
             v["achieved_TFLOPs"] = round(tf, 2)
             v["valu_frac"] = round(tf / VALU_PEAK_TFLOPS, 4)
-        if k in traffic:
+        if k in traffic and "bytes_per_launch" in traffic[k]:
             v["traffic_bytes_per_launch"] = traffic[k]["bytes_per_launch"]
+        if k in traffic and "valu_wave_instr_per_launch" in traffic[k]:
+            # one wavefront VALU instruction per cycle per CU (4 SIMDs, each
+            # issuing a wave64 instruction every 4 cycles) at 2.4 GHz
+            issue_ms = traffic[k]["valu_wave_instr_per_launch"] / (CUS * CLOCK_GHZ * 1e9) * 1e3
+            v["valu_issue_ms"] = round(issue_ms, 4)
+            v["valu_issue_frac"] = round(issue_ms / v["ms_per_launch"], 4)
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_launch"])
     kd = kernels[dom]
     tr = traffic.get(dom)
     roof = {"bound": "hbm", "kernel": kernel_names(*c["order"])[dom], "achieved": kd["achieved_GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": kd["hbm_frac"],
             "frac_of_copy_ceiling": round(kd["achieved_GBs"] / ceiling["best_GBs"]["copy"], 4),
-            "traffic": tr["bytes_per_launch"] if tr else None,
+            "traffic": tr.get("bytes_per_launch") if tr else None,
+            "valu_issue_frac": kd.get("valu_issue_frac"),
             "algorithmic_bytes_per_launch": kd["bytes_per_cell"] * ncell, "ms_per_launch": kd["ms_per_launch"]}
     # the whole step against its algorithmic bytes: 186 field passes per
     # moist SSP-RK3 step (62 per stage), FP32 storage

@@ -199,24 +199,29 @@ template <typename T> struct Interp<5, T> {
         return w0 * s0 + w1 * s1 + w2 * s2;
     }
     // weno() of a stencil whose centre value q[2] is zero (offsets from the
-    // centre): the same polynomials and smoothness indicators with the
-    // centre terms dropped.
+    // centre), for the FP32 form, rearranged for fewer operations (the same
+    // function in exact arithmetic): the linear weights d = (0.1, 0.6, 0.3)
+    // are folded into the candidate polynomials, the smoothness indicators
+    // are taken times 4 with epsilon times 4 (the weights are unchanged by a
+    // common factor), and the weighted sum and the sum of the weights are two
+    // multiply-add chains over r_k = 1 / (4 b_k + 4 eps)^2 with one final
+    // reciprocal: 34 operations and 4 reciprocals against 43 and 4.
     static __device__ __forceinline__ T weno_c0(const T *q) {
         const T m2 = q[0], m1 = q[1], p1 = q[3], p2 = q[4];
-        const T s0 = T(1.0 / 3.0) * m2 - T(7.0 / 6.0) * m1;
-        const T s1 = T(-1.0 / 6.0) * m1 + T(1.0 / 3.0) * p1;
-        const T s2 = T(5.0 / 6.0) * p1 - T(1.0 / 6.0) * p2;
+        const T s0 = T(0.1 / 3.0) * m2 - T(0.7 / 6.0) * m1;  // 0.1 ((1/3) m2 - (7/6) m1)
+        const T s1 = T(-0.1) * m1 + T(0.2) * p1;             // 0.6 (-(1/6) m1 + (1/3) p1)
+        const T s2 = T(0.25) * p1 - T(0.05) * p2;            // 0.3 ((5/6) p1 - (1/6) p2)
         const T e2 = p2 - T(2.0) * p1, f2 = p2 - T(4.0) * p1;
         const T e1 = m1 + p1, f1 = m1 - p1;
         const T e0 = m2 - T(2.0) * m1, f0 = m2 - T(4.0) * m1;
-        const T b2 = T(13.0 / 12.0) * e2 * e2 + T(0.25) * f2 * f2;
-        const T b1 = T(13.0 / 12.0) * e1 * e1 + T(0.25) * f1 * f1;
-        const T b0 = T(13.0 / 12.0) * e0 * e0 + T(0.25) * f0 * f0;
-        const T a0 = Num<T>::div(T(0.1), sq_eps(b0));
-        const T a1 = Num<T>::div(T(0.6), sq_eps(b1));
-        const T a2 = Num<T>::div(T(0.3), sq_eps(b2));
-        const T inv = Num<T>::rcp(a0 + a1 + a2);
-        return (a0 * s0 + a1 * s1 + a2 * s2) * inv;
+        const T B2 = (T(13.0 / 3.0) * e2) * e2 + f2 * f2;  // 4 b2
+        const T B1 = (T(13.0 / 3.0) * e1) * e1 + f1 * f1;
+        const T B0 = (T(13.0 / 3.0) * e0) * e0 + f0 * f0;
+        const T t0 = B0 + T(4e-10), t1 = B1 + T(4e-10), t2 = B2 + T(4e-10);
+        const T r0 = Num<T>::rcp(t0 * t0), r1 = Num<T>::rcp(t1 * t1), r2 = Num<T>::rcp(t2 * t2);
+        const T num = s0 * r0 + s1 * r1 + s2 * r2;
+        const T den = T(0.1) * r0 + T(0.6) * r1 + T(0.3) * r2;
+        return num * Num<T>::rcp(den);
     }
     // interp_4: q = phim1, phi, phip1, phip2
     static __device__ __forceinline__ T centred(const T *q) {
