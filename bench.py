@@ -145,7 +145,11 @@ def field_errors(sim32, sim64):
         d = np.abs(a - b)
         if n in ("temperature", "qt", "s"):
             rel = d / np.abs(b)
-            out[n] = {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel * rel)))}
+            # where the error sits: the 99.99th percentile and the share of
+            # cells beyond the 1e-5 gate, beside the maximum and RMS
+            out[n] = {"max_rel": float(rel.max()), "rms_rel": float(np.sqrt(np.mean(rel * rel))),
+                      "p9999_rel": float(np.quantile(rel, 0.9999)),
+                      "frac_above_1e-5": float(np.count_nonzero(rel > 1e-5)) / rel.size}
         else:
             scale = float(np.max(np.abs(b)))
             out[n] = {"max_abs": float(d.max()), "rms_abs": float(np.sqrt(np.mean(d * d))),

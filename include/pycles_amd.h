@@ -90,11 +90,16 @@ int pycles_device_count(void);
  * are zero on entry (the w tendency is still read: it carries buoyancy); they
  * are then written instead of read-modified-written, with identical bits.
  * PYCLES_ADV_PASS_X/Y/Z: when any is set, only the selected directional
- * passes run (for timing one kernel; a partial call is not a full update). */
+ * passes run (for timing one kernel; a partial call is not a full update).
+ * The y and z passes run as one fused kernel when both are selected, the
+ * orders are 5 and 5 and the call has momentum and at least one scalar: by
+ * default for FP32 storage; PYCLES_ADV_FUSE_YZ asks for it in FP64 as well
+ * (same bits as the separate passes; for tests, it is slower there). */
 #define PYCLES_ADV_TEND_ZERO 1
 #define PYCLES_ADV_PASS_X 2
 #define PYCLES_ADV_PASS_Y 4
 #define PYCLES_ADV_PASS_Z 8
+#define PYCLES_ADV_FUSE_YZ 16
 int pycles_advection(const pycles_dims *dims, const pycles_ref *ref, const void *const vel[3],
                      void *const tend_vel[3], int nscal, const void *const *scal,
                      void *const *tend_scal, const pycles_scalar_base *base, int scalar_order,

@@ -51,6 +51,7 @@ cdef extern from "pycles_amd.h":
     int PYCLES_ADV_PASS_X
     int PYCLES_ADV_PASS_Y
     int PYCLES_ADV_PASS_Z
+    int PYCLES_ADV_FUSE_YZ
     int PYCLES_DIFF_SCALARS
     int PYCLES_DIFF_MOMENTUM
     int PYCLES_DIFF_ENTROPY_SOURCE
@@ -154,6 +155,7 @@ ADV_TEND_ZERO = PYCLES_ADV_TEND_ZERO
 ADV_PASS_X = PYCLES_ADV_PASS_X
 ADV_PASS_Y = PYCLES_ADV_PASS_Y
 ADV_PASS_Z = PYCLES_ADV_PASS_Z
+ADV_FUSE_YZ = PYCLES_ADV_FUSE_YZ
 DIFF_SCALARS = PYCLES_DIFF_SCALARS
 DIFF_MOMENTUM = PYCLES_DIFF_MOMENTUM
 DIFF_ENTROPY_SOURCE = PYCLES_DIFF_ENTROPY_SOURCE

@@ -56,6 +56,7 @@ static int advection_run(const pycles_dims *dims, const pycles_ref *ref, const v
     fl.tzero = (flags & PYCLES_ADV_TEND_ZERO) ? 1 : 0;
     fl.passes = (flags >> 1) & 7;
     if (!fl.passes) fl.passes = 7;
+    fl.fuse_yz = (flags & PYCLES_ADV_FUSE_YZ) ? 1 : 0;
     for (int s = 0; s < nscal; ++s) {
         fl.q[s] = scal[s];
         fl.t[s] = tend_scal[s];

@@ -26,13 +26,6 @@
 
 #include "common.h"
 
-// rows per block of the fused y-z pass
-#ifndef PYCLES_YZ_ROWS
-#define PYCLES_YZ_ROWS 4
-#endif
-#ifndef PYCLES_YZ_WAVES
-#define PYCLES_YZ_WAVES 3
-#endif
 
 namespace adv {
 
@@ -56,6 +49,7 @@ struct Fields {
     const double *base[2];
     int tzero;  // scalar, u, v tendencies are zero on entry (x pass writes them)
     int passes;  // bit 0: x, bit 1: y, bit 2: z
+    int fuse_yz;  // run the y and z passes as one kernel in FP64 too (FP32: always, where the orders allow)
     // RK3 stage fused into the z pass (rk >= 0): new values to out, stage
     // copies in cp, z boundary fill (sym for scalars, u, v; asym for w)
     int rk;
@@ -103,8 +97,9 @@ struct PolicyF64 {
     using TC = double;
     using SFlux = double;
     static constexpr bool kBase = false;
-    // the y and z passes stay separate: the fused kernel's double-buffered
-    // FP64 tiles (86 KB of LDS) leave one block per CU
+    // the y and z passes stay separate unless asked (PYCLES_ADV_FUSE_YZ, for
+    // tests): the fused kernel's double-buffered FP64 tiles (86 KB of LDS)
+    // leave one block per CU
     static constexpr bool kFuseYZ = false;
     static constexpr int kYZWaves = 1;
     template <int O, bool BV>
@@ -154,7 +149,7 @@ struct PolicyF32 {
     using SFlux = SF32;
     static constexpr bool kBase = true;
     static constexpr bool kFuseYZ = true;  // y and z passes fused (WENO5)
-    static constexpr int kYZWaves = PYCLES_YZ_WAVES;     // waves per SIMD of the fused pass
+    static constexpr int kYZWaves = 3;     // waves per SIMD of the fused pass (its LDS allows 3 blocks per CU)
     template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *bwin, TC U, double rho) {
         constexpr int R = Interp<O, float>::R;
@@ -1253,9 +1248,12 @@ int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
         PYCLES_CHECK_LAUNCH();
     }
     // y and z fused (WENO5 with momentum and scalars), z-chunked like the z pass
-    if constexpr (P::kFuseYZ && SO == 5 && MO == 5 && MOM && NSC >= 1) {
-        if ((fl.passes & 6) == 6) {
-            constexpr int TY = PYCLES_YZ_ROWS;
+    if constexpr (SO == 5 && MO == 5 && MOM && NSC >= 1) {
+        if ((fl.passes & 6) == 6 && (P::kFuseYZ || fl.fuse_yz)) {
+            // 4 rows per block: 8 rows halve the lower faces' extra work but
+            // need more registers than 4 waves per SIMD allow with the RK3
+            // stage (spills), and at 3 waves a 512-thread block fits once per CU
+            constexpr int TY = 4;
             ZSplit zs = advz_zsplit(g.nx, g.ny, g.nz);
             if (zs.len > YZ_KMAX) zs = zsplit_ceil(g.nz, (g.nz + YZ_KMAX - 1) / YZ_KMAX);
             const dim3 grid((g.nx + 63) / 64, (g.ny + TY - 1) / TY, zs.nch), blk(64, TY);

@@ -42,18 +42,21 @@ def test_advection_bitwise(gpu, order):
     nl, Gr, RS, Th, fields = host_case(24, 20, 16, gw=gw, order=order, seed=7)
     fields = _swirl(Gr, fields)
     sim = _gpu_sim(nl, Gr, RS, Th, fields)
+    from pycles_amd import _capi
     from pycles_amd.Advection import advect_fused
-    sim.PV.zero_tendencies()
-    advect_fused(sim.SA, sim.MA, sim.Gr, sim.Ref, sim.PV)
-    torch.cuda.synchronize()
     osim = oracle_sim(Gr, RS, Th, fields, True, order=order)
     osim.scalar_advection_update()
     osim.momentum_advection_update()
     gw = Gr.dims.gw
-    for n in ("u", "v", "w", "s", "qt"):
-        g = sim.PV.get_tendency_array(n, Gr)[gw:-gw, gw:-gw, gw:-gw]
-        o = osim.pt(n)[gw:-gw, gw:-gw, gw:-gw]
-        assert np.array_equal(g, o), f"{n}: max diff {np.abs(g - o).max()} (rel {rel_err(g, o)})"
+    # WENO5: also the fused y-z kernel, which FP64 runs only when asked
+    for flags in ((0, _capi.ADV_FUSE_YZ) if order == (5, 5) else (0,)):
+        sim.PV.zero_tendencies()
+        advect_fused(sim.SA, sim.MA, sim.Gr, sim.Ref, sim.PV, passes=flags)
+        torch.cuda.synchronize()
+        for n in ("u", "v", "w", "s", "qt"):
+            g = sim.PV.get_tendency_array(n, Gr)[gw:-gw, gw:-gw, gw:-gw]
+            o = osim.pt(n)[gw:-gw, gw:-gw, gw:-gw]
+            assert np.array_equal(g, o), f"{n} (flags {flags}): max diff {np.abs(g - o).max()} (rel {rel_err(g, o)})"
 
 
 def test_advection_separate_equals_fused(gpu):
@@ -74,9 +77,10 @@ def test_advection_separate_equals_fused(gpu):
 @pytest.mark.parametrize("shape", [(24, 20, 16), (64, 8, 97), (96, 13, 33)])
 def test_fused_yz_matches_separate_passes(gpu, precision, shape):
     """One advection call runs the x pass and then the fused y-z kernel
-    (WENO5); the x, y and z passes launched one at a time give the same bits
-    in FP64, for the tendencies and for the RK3 stage fused into the z march
-    (every stage, ghost levels included).  The FP32 kernels are compiled with
+    (WENO5; FP32 by default, FP64 when asked with PYCLES_ADV_FUSE_YZ); the x,
+    y and z passes launched one at a time give the same bits in FP64, for the
+    tendencies and for the RK3 stage fused into the z march (every stage,
+    ghost levels included).  The FP32 kernels are compiled with
     multiply-adds fused, which the compiler forms per kernel, so there the two
     agree to rounding: within 1e-6 of each field's largest tendency (~8 ulps;
     measured up to 1.7e-7) and of its largest value plus dt times that.  The
@@ -90,7 +94,7 @@ def test_fused_yz_matches_separate_passes(gpu, precision, shape):
     nl["mi355x"]["precision"] = precision
     sim = _gpu_sim(nl, Gr, RS, Th, _swirl(Gr, fields))
     PV = sim.PV
-    X, Y, Z = _capi.ADV_PASS_X, _capi.ADV_PASS_Y, _capi.ADV_PASS_Z
+    X, Y, Z, FUSE = _capi.ADV_PASS_X, _capi.ADV_PASS_Y, _capi.ADV_PASS_Z, _capi.ADV_FUSE_YZ
     print(f"fused y-z: {precision} {shape}", flush=True)
 
     def tendencies(seq):
@@ -108,7 +112,7 @@ def test_fused_yz_matches_separate_passes(gpu, precision, shape):
             d = (a[v].double() - b[v].double()).abs().max().item()
             assert d <= 1e-6 * scale[v], (what, n, d, scale[v])
 
-    fused, sep = tendencies([0]), tendencies([X, Y, Z])
+    fused, sep = tendencies([FUSE]), tendencies([X, Y, Z])
     tscale = [sep[v].double().abs().max().item() for v in range(len(PV.index_name))]
     agree(fused, sep, tscale, "tendencies")
     cp = PV.values.clone()
@@ -117,7 +121,7 @@ def test_fused_yz_matches_separate_passes(gpu, precision, shape):
     for rk in range(3):
         sim.TS.rk_step = rk
         outs = []
-        for seq in ([0], [X, Y, Z]):
+        for seq in ([FUSE], [X, Y, Z]):
             PV.zero_tendencies()
             out = torch.zeros_like(PV.values)
             for p in seq:
