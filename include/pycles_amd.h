@@ -357,7 +357,8 @@ int pycles_cloud_stats(const pycles_dims *dims, const double *rho0_half, const v
  * the remainder: chunk b is [gw + b*len, b+1 == n ? nz+gw : gw + (b+1)*len)),
  * 1 = advection y pass (rows [0, ny), chunk b is [b*len, min((b+1)*len, ny))),
  * 2 = advection z pass (interior levels, as 1 from gw), 3 = thermodynamics
- * (all nzg levels, as 1).  No reference counterpart. */
+ * (all nzg levels, as 1), 4 = the fused y-z advection pass (as 2, no chunk
+ * longer than 128 levels).  No reference counterpart. */
 int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nchunks, int *chunk_len);
 
 /* ---- HBM calibration (no reference counterpart) ------------------------ */

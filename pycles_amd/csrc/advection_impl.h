@@ -727,10 +727,9 @@ __global__ __launch_bounds__(256) void adv_y(Geo g, Prof pr, Fields fl, int jchu
 // read and write and its re-read of the fields.  Columns beyond a partial
 // tile (nx not a multiple of 64, ny of TY; PART) march wrapped copies of
 // real columns and store nothing, so every thread reaches every barrier.
-// A chunk holds at most YZ_KMAX levels: its level profiles are staged in LDS
-// (read from global memory they are vector loads, whose waits would drain the
-// prefetches in flight).
-constexpr int YZ_KMAX = 128;
+// A chunk holds at most ADV_YZ_KMAX levels (advyz_zsplit): its level
+// profiles are staged in LDS (read from global memory they are vector loads,
+// whose waits would drain the prefetches in flight).
 
 template <class P, int SO, int MO, int NSC, bool MOM, int RKM, int TY, bool PART>
 __global__ __launch_bounds__(64 * TY, P::kYZWaves) void adv_yz(Geo g, Prof pr, Fields fl, int kchunk) {
@@ -749,7 +748,7 @@ __global__ __launch_bounds__(64 * TY, P::kYZWaves) void adv_yz(Geo g, Prof pr, F
     constexpr int FU = NSC, FV = NSC + 1, FW = NSC + 2;  // tile field index
     constexpr int NHR = (2 * R + TY - 1) / TY;           // halo rows per thread row
     constexpr int NXH = (2 * TY + 1) * (NC - 1);         // x halo items (v: TY + 1 rows, w: TY rows)
-    constexpr int NPL = YZ_KMAX + 2 * R;                 // profile levels k0 - R .. k0 + kchunk + R - 1
+    constexpr int NPL = ADV_YZ_KMAX + 2 * R;             // profile levels k0 - R .. k0 + kchunk + R - 1
     constexpr int NPR = P::kBase ? 4 + NSC : 4;
     static_assert(NXH <= 64 * TY, "x halo items exceed the block");
     // tile and face fluxes double-buffered by level parity: level c + 1 is
@@ -1254,8 +1253,7 @@ int launch(const Geo &g, const Prof &pr, const Fields &fl, hipStream_t st) {
             // need more registers than 4 waves per SIMD allow with the RK3
             // stage (spills), and at 3 waves a 512-thread block fits once per CU
             constexpr int TY = 4;
-            ZSplit zs = advz_zsplit(g.nx, g.ny, g.nz);
-            if (zs.len > YZ_KMAX) zs = zsplit_ceil(g.nz, (g.nz + YZ_KMAX - 1) / YZ_KMAX);
+            const ZSplit zs = advyz_zsplit(g.nx, g.ny, g.nz);
             const dim3 grid((g.nx + 63) / 64, (g.ny + TY - 1) / TY, zs.nch), blk(64, TY);
             const bool part = g.nx % 64 != 0 || g.ny % TY != 0;
             const int rkm = fl.rk < 0 ? 0 : (fl.rk == 0 ? 1 : 2);

@@ -97,6 +97,13 @@ static inline ZSplit advz_zsplit(int nx, int ny, int nz) {
     while (cols * want < 2048 && nz / (want * 2) >= 32) want *= 2;
     return zsplit_ceil(nz, want);
 }
+// The fused y-z advection pass: the z pass's chunks, none longer than
+// ADV_YZ_KMAX levels (the LDS copy of a chunk's level profiles holds that many).
+constexpr int ADV_YZ_KMAX = 128;
+static inline ZSplit advyz_zsplit(int nx, int ny, int nz) {
+    const ZSplit z = advz_zsplit(nx, ny, nz);
+    return z.len > ADV_YZ_KMAX ? zsplit_ceil(nz, (nz + ADV_YZ_KMAX - 1) / ADV_YZ_KMAX) : z;
+}
 // The thermodynamics kernels' chunks of the nzg ghosted levels (~4096 blocks;
 // their LDS holds len + 2 levels, so no chunk is longer than len).
 static inline ZSplit thermo_zsplit(int nx, int ny, int nzg) {

@@ -489,6 +489,7 @@ extern "C" int pycles_launch_chunks(int kernel, const pycles_dims *dims, int *nc
     case 1: z = advy_zsplit(g.nx, g.ny, g.nz); break;
     case 2: z = advz_zsplit(g.nx, g.ny, g.nz); break;
     case 3: z = thermo_zsplit(g.nx, g.ny, g.nzg); break;
+    case 4: z = advyz_zsplit(g.nx, g.ny, g.nz); break;
     default:
         pycles_set_error("launch_chunks: unknown kernel %d", kernel);
         return PYCLES_ERR_ARG;

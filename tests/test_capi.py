@@ -110,7 +110,9 @@ def test_z_chunks_partition_every_level_count(lib):
     """The z chunks of the chunked kernels, as their grids use them (the
     formulae in the header), cover each level exactly once with no empty
     chunk; the pressure correction's chunks hold at least gw levels each
-    (its first and last chunks fill the ghost levels from them)."""
+    (its first and last chunks fill the ghost levels from them); the fused
+    y-z pass's chunks hold at most the 128 levels its LDS profile copy
+    holds."""
     for gw in (3, 5):
         for plane in ((64, 64), (512, 512), (16, 8)):
             for nz in range(max(2, gw), 2049):
@@ -121,10 +123,11 @@ def test_z_chunks_partition_every_level_count(lib):
                 assert all(r[1] == q[0] for r, q in zip(ranges, ranges[1:]))
                 lens = [r[1] - r[0] for r in ranges]
                 assert min(lens) >= (gw if nch > 1 else 1), (gw, n, ranges)
-                for kern, total in ((1, plane[1]), (2, nz), (3, nz + 2 * gw)):
+                for kern, total in ((1, plane[1]), (2, nz), (3, nz + 2 * gw), (4, nz)):
                     nch, ln = _chunks(lib, kern, n, gw)
                     ranges = [(b * ln, min((b + 1) * ln, total)) for b in range(nch)]
                     assert ranges[-1][1] == total and all(r[1] > r[0] for r in ranges), (kern, n, ranges)
+                    assert kern != 4 or ln <= 128, (n, ln)
 
 
 def test_layout_and_cfl_read_reject_bad_arguments(lib):
