@@ -200,11 +200,15 @@ class FP32Storage:
                 f.copy_(f.float().double())
 
 
-def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True, floor_kinds=("ulp64", "fp32", "storage")):
+def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True,
+                floor_kinds=("ulp64", "fp32", "storage", "secant")):
     """FP32 + learned against FP64 from the same state after 1 and nsteps
     steps; with `floors`, also FP64 runs from the perturbed states of
-    `perturbed_fields` and an FP64 run rounded to FP32 storage after every
-    step (FP32Storage) against the same FP64 run (their noise floors)."""
+    `perturbed_fields`, an FP64 run rounded to FP32 storage after every
+    step (FP32Storage), and an FP64 run whose saturation-adjustment secant
+    stops at |dT| < 1e-7 K instead of the reference's 1e-3 K (how far the
+    reference's own stopping point moves the run), each against the same
+    FP64 run (their noise floors)."""
     import torch
     runs = {"fp32_learned": make_sim(nl, Gr, RS, Th, fields, "f32", learned),
             "fp64": make_sim(nl, Gr, RS, Th, fields, "f64")}
@@ -216,13 +220,18 @@ def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True, floor_kind
         if "storage" in floor_kinds:
             runs["fp64_fp32_storage"] = make_sim(nl, Gr, RS, Th, perturbed_fields(fields, "fp32"), "f64")
             storage = FP32Storage(runs["fp64_fp32_storage"], fields)
+        if "secant" in floor_kinds:
+            runs["fp64_secant_1e-7"] = make_sim(nl, Gr, RS, Th, fields, "f64")
     out = {"errors_vs_fp64": {}, "noise_floor_fp64": {}}
     for name, sim in runs.items():
         sim.Th.mlp = learned if name == "fp32_learned" else None
+        sim.Th.secant_tol = 1e-7 if name == "fp64_secant_1e-7" else 0.0
         sim.begin_run()
+    # the runs share Th: its network and secant tolerance are set per run
     for n in range(1, nsteps + 1):
         for name, sim in runs.items():
             sim.Th.mlp = learned if name == "fp32_learned" else None
+            sim.Th.secant_tol = 1e-7 if name == "fp64_secant_1e-7" else 0.0
             sim.step()
         if storage is not None:
             storage.round()
@@ -231,36 +240,47 @@ def error_study(nl, Gr, RS, Th, fields, learned, nsteps, floors=True, floor_kind
             key = f"after_{n}_steps"
             out["errors_vs_fp64"][key] = field_errors(runs["fp32_learned"], runs["fp64"])
             for name in runs:
-                if name.startswith("fp64_from_") or name == "fp64_fp32_storage":
+                if name.startswith("fp64_") and name != "fp64":
                     out["noise_floor_fp64"].setdefault(name, {})[key] = field_errors(runs[name], runs["fp64"])
             e = out["errors_vs_fp64"][key]
             log(f"[bench] errors after {n} steps: T {e['temperature']} qt {e['qt']}")
             for name, fl in out["noise_floor_fp64"].items():
                 log(f"[bench] floor {name} after {n} steps: T {fl[key]['temperature']} qt {fl[key]['qt']}")
     runs["fp32_learned"].Th.mlp = None
+    runs["fp32_learned"].Th.secant_tol = 0.0
     return out
 
 
-def error_gate(errors, tol=1e-5, floor=None):
+def error_gate(errors, tol=1e-5, floor=None, secant_floor=None):
     """The metric's gate: T and qt within tol (max relative) of FP64.  With
     `floor` (the FP32-storage run's errors), also the gate against what FP32
-    storage alone gives: within max(tol, 2 x floor)."""
+    storage alone gives: within max(tol, 2 x floor).  With `secant_floor`
+    (the errors of an FP64 run whose saturation adjustment converges to
+    1e-7 K instead of stopping at the reference's 1e-3 K), the gate against
+    the reference method's own resolution: within max(tol, 2 x that)."""
     out = {"tolerance": tol, "fields": ["temperature", "qt"], "failing": []}
+    gates = {}
     if floor:
-        out["floor_gate"] = {"rule": "max_rel <= max(tol, 2 x FP64 rounded to FP32 storage every step)",
-                             "failing": []}
+        gates["floor_gate"] = ("max_rel <= max(tol, 2 x FP64 rounded to FP32 storage every step)", floor)
+    if secant_floor:
+        gates["reference_stopping_gate"] = (
+            "max_rel <= max(tol, 2 x FP64 with its secant converged to 1e-7 K instead of the reference's 1e-3 K)",
+            secant_floor)
+    for g, (rule, _) in gates.items():
+        out[g] = {"rule": rule, "failing": []}
     for key, e in errors.items():
         for f in out["fields"]:
             v = e[f]["max_rel"]
             if not v <= tol:
                 out["failing"].append(f"{f} {key}: {v:.3g}")
-            if floor and key in floor:
-                lim = max(tol, 2.0 * floor[key][f]["max_rel"])
-                if not v <= lim:
-                    out["floor_gate"]["failing"].append(f"{f} {key}: {v:.3g} > {lim:.3g}")
+            for g, (_, fl) in gates.items():
+                if key in fl:
+                    lim = max(tol, 2.0 * fl[key][f]["max_rel"])
+                    if not v <= lim:
+                        out[g]["failing"].append(f"{f} {key}: {v:.3g} > {lim:.3g}")
     out["pass"] = not out["failing"]
-    if floor:
-        out["floor_gate"]["pass"] = not out["floor_gate"]["failing"]
+    for g in gates:
+        out[g]["pass"] = not out[g]["failing"]
     return out
 
 
@@ -565,7 +585,8 @@ def main():
         es = error_study(nl, Gr, RS, Th, fields, guess, args.error_steps, floors=not args.no_floor)
         out["errors_vs_fp64"] = es["errors_vs_fp64"]
         out["error_gate"] = error_gate(es["errors_vs_fp64"],
-                                       floor=es["noise_floor_fp64"].get("fp64_fp32_storage"))
+                                       floor=es["noise_floor_fp64"].get("fp64_fp32_storage"),
+                                       secant_floor=es["noise_floor_fp64"].get("fp64_secant_1e-7"))
         if es["noise_floor_fp64"]:
             out["noise_floor_fp64"] = es["noise_floor_fp64"]
         torch.cuda.empty_cache()

@@ -211,6 +211,9 @@ typedef struct pycles_sa_options {
     int newton_steps;    /* Newton corrections after the learned guess, 1..4 */
     long long *counters; /* device int64[4] or NULL */
     int wt_overwrite;    /* 1: the w tendency is zero on entry, write instead of add */
+    double secant_tol;   /* the FP64 secant's stopping test |dT| < secant_tol (K); 0: the
+                            reference's 1e-3 K.  Smaller values measure how far the
+                            reference's own stopping moves a run (no reference counterpart) */
 } pycles_sa_options;
 
 /* Replaces ThermodynamicsSA.update (ThermodynamicsSA.pyx:176-213):

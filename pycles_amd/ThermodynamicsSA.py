@@ -33,6 +33,10 @@ class ThermodynamicsSA:
         # 1-step T error against FP64 falls from 9.5e-7 to 2.1e-7 relative)
         # and leaves no cell to the fallback, for ~1% of the step
         self.newton_steps = int(th.get("newton_steps", 2))
+        # the FP64 secant's stopping test |dT| < secant_tol; 0 keeps the
+        # reference's 1e-3 K (a smaller value is for measuring how far that
+        # stopping point moves a run)
+        self.secant_tol = float(th.get("secant_tol", 0.0))
 
     def initialize(self, Gr, PV, DV, NS=None, Pa=None):
         PV.add_variable("s", "J kg^-1 K^-1", "s", "specific entropy", "sym", "scalar", Pa)
@@ -115,7 +119,7 @@ class ThermodynamicsSA:
     def options(self, wt_overwrite=False):
         return _capi.SAOptions(self.do_qt_clipping, self.max_iter, _capi.LH_VARIABLE,
                                None if self.mlp is None else self.mlp.handle_ptr(),
-                               self.newton_steps, self.counters.data_ptr(), wt_overwrite)
+                               self.newton_steps, self.counters.data_ptr(), wt_overwrite, self.secant_tol)
 
     def update(self, Gr, RS, PV, DV, wt_overwrite=False):
         """wt_overwrite: the w tendency is zero (or stale and to be replaced),

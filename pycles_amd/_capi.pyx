@@ -40,6 +40,7 @@ cdef extern from "pycles_amd.h":
         int newton_steps
         long long *counters
         int wt_overwrite
+        double secant_tol
     ctypedef struct pycles_pressure_plan:
         pass
 
@@ -230,7 +231,7 @@ cdef class SAOptions:
     cdef pycles_sa_options o
 
     def __init__(self, do_qt_clipping=True, max_iter=100, latent_heat=0, mlp=None,
-                 newton_steps=0, counters=None, wt_overwrite=False):
+                 newton_steps=0, counters=None, wt_overwrite=False, secant_tol=0.0):
         self.o.do_qt_clipping = 1 if do_qt_clipping else 0
         self.o.max_iter = max_iter
         self.o.latent_heat = latent_heat
@@ -238,6 +239,7 @@ cdef class SAOptions:
         self.o.newton_steps = newton_steps
         self.o.counters = <long long *> _p(counters)
         self.o.wt_overwrite = 1 if wt_overwrite else 0
+        self.o.secant_tol = secant_tol
 
 
 cdef class PressurePlan:

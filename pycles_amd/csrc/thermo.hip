@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
                 }
             }
             double T, qv, ql, qi;
-            const int it = thermo::eos_sa(lt, p0, sk, qt, &T, &qv, &ql, &qi, a.max_iter);
+            const int it = thermo::eos_sa(lt, p0, sk, qt, &T, &qv, &ql, &qi, a.max_iter, a.sa_tol);
             if (it > 0 && own) {
                 ctr[0] += 1;
                 ctr[1] += it;
@@ -185,6 +185,7 @@ extern "C" int pycles_thermo_sa(const pycles_dims *dims, const pycles_ref *ref, 
     a.wt_over = opt->wt_overwrite;
     a.mlp = (const mlp::Weights *)opt->mlp;
     a.newton = opt->newton_steps;
+    a.sa_tol = opt->secant_tol > 0.0 ? opt->secant_tol : 1.0e-3;
     a.counters = (unsigned long long *)opt->counters;
     const ZSplit zs = thermo_zsplit(g.nx, g.ny, g.nzg);
     a.kchunk = zs.len;

@@ -20,6 +20,7 @@ struct SAArgs {
     int newton;
     unsigned long long *counters;
     int kchunk;
+    double sa_tol;  // the FP64 secant's stopping test (the reference: 1e-3 K)
 };
 
 // Block-level reduction of the four counters into out.

@@ -75,10 +75,11 @@ __device__ __forceinline__ double temperature_no_ql(double pd, double pv, double
 }
 
 // eos_c, thermodynamics_sa.h:17-66, lambda == 1; capped at max_iter secant
-// iterations (max_iter <= 0: uncapped as in the reference).  Returns the
-// number of secant iterations (0 = unsaturated closed form).
+// iterations (max_iter <= 0: uncapped as in the reference); the stopping test
+// is |dT| < tol (the reference's 1e-3 K by default).  Returns the number of
+// secant iterations (0 = unsaturated closed form).
 __device__ __forceinline__ int eos_sa(const Table &lt, double p0, double s, double qt, double *T, double *qv,
-                                      double *ql, double *qi, int max_iter) {
+                                      double *ql, double *qi, int max_iter, double tol = 1.0e-3) {
     *qv = qt;
     *ql = 0.0;
     *qi = 0.0;
@@ -114,7 +115,7 @@ __device__ __forceinline__ int eos_sa(const Table &lt, double p0, double s, doub
         delta_T = fabs(T_2 - T_1);
         ++it;
         if (max_iter > 0 && it >= max_iter) break;
-    } while (delta_T >= 1.0e-3 || sigma_2 < 0.0);
+    } while (delta_T >= tol || sigma_2 < 0.0);
     *T = T_2;
     *qv = qv_star_2;
     *ql = 1.0 * sigma_2;
