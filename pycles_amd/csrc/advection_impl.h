@@ -1098,21 +1098,47 @@ __global__ __launch_bounds__(256) void adv_x(Geo g, Prof pr, Fields fl) {
     auto field = [](int f) { return f < NSC ? f : VU + (f - NSC); };
 
     // phase 1, in batches whose loads are all issued before the LDS writes
-    // (a load written to LDS inside a plain loop waits once per element)
-    for (int t0 = tid; t0 < W; t0 += 4 * B) {
-        TC tmp[NV][4];
+    // (a load written to LDS inside a plain loop waits once per element).
+    // Even nx: the row's cells in pairs (one 2-vector load per field and
+    // pair, no wrap arithmetic), then the 2H periodic halo cells; otherwise
+    // element by element with the wrap.
+    if ((nx & 1) == 0) {
+        using TS2 = typename std::conditional<sizeof(TS) == 4, float2, double2>::type;
+        for (int e = 2 * tid; e < nx; e += 2 * B) {
+            TS2 tmp[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
+            for (int v = 0; v < NV; ++v) tmp[v] = *reinterpret_cast<const TS2 *>(src[v] + row + e);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int t = t0 + u * B;
-                tmp[v][u] = t < W ? (TC)ld(src[v], row + wrapi(t - H, nx)) : TC(0);
+            for (int v = 0; v < NV; ++v) {
+                rows[v * W + H + e] = (TC)tmp[v].x;
+                rows[v * W + H + e + 1] = (TC)tmp[v].y;
             }
+        }
+        for (int h = tid; h < 2 * H; h += B) {
+            const int si = h < H ? nx - H + h : h - H;  // left halo: the row's end; right: its start
+            const int di = h < H ? h : nx + h;
+            TC tmp[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
+            for (int v = 0; v < NV; ++v) tmp[v] = (TC)ld(src[v], row + si);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (t0 + u * B < W) rows[v * W + t0 + u * B] = tmp[v][u];
+            for (int v = 0; v < NV; ++v) rows[v * W + di] = tmp[v];
+        }
+    } else {
+        for (int t0 = tid; t0 < W; t0 += 4 * B) {
+            TC tmp[NV][4];
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = t0 + u * B;
+                    tmp[v][u] = t < W ? (TC)ld(src[v], row + wrapi(t - H, nx)) : TC(0);
+                }
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (t0 + u * B < W) rows[v * W + t0 + u * B] = tmp[v][u];
+        }
     }
     const int nt = L::nt(tzero);
     const int f0 = NF - nt;  // the staged tendencies are those of the last nt variables
