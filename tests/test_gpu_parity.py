@@ -176,16 +176,14 @@ def test_thermo_sa_parity(gpu):
     g = sim.DV.get_device_layout("buoyancy")
     assert np.abs(g[ok] - o[ok]).max() <= 1e-12 * 9.80665
     # N^2 uses centred differences of T (or theta_rho) over neighbours:
-    # compare where no neighbour flipped.  Each T is within ~1e-12 relative
-    # (3e-10 K), so a two-level difference carries up to ~6e-10 K against
-    # differences of 0.1-1 K in this state, and the saturated branch's
-    # dT/dz + Gamma_w cancels further: 1e-10 normwise bounds that
+    # compare where no neighbour flipped, at the 1e-12 of the other fields
+    # (measured 6e-15 normwise, no flips in 35328 cells)
     nb = flip | np.roll(flip, 1, 0) | np.roll(flip, -1, 0)
     o = Gr.to_device_layout(osim.dv["buoyancy_frequency"])
     g = sim.DV.get_device_layout("buoyancy_frequency")
     print(f"N^2 normwise relative error {rel_err(g[~nb], o[~nb]):.3e}; secant flips {int(flip.sum())} "
           f"of {T_o.size}", flush=True)
-    assert rel_err(g[~nb], o[~nb]) <= 1e-10, rel_err(g[~nb], o[~nb])
+    assert rel_err(g[~nb], o[~nb]) <= 1e-12, rel_err(g[~nb], o[~nb])
     wt_o = Gr.to_device_layout(osim.pt("w"))[gw:-gw]
     wt_g = sim.PV.tend("w").to(torch.float64).cpu().numpy()[gw:-gw]
     okw = ~(nb[gw:-gw] | np.roll(nb, -1, 0)[gw:-gw])
