@@ -102,6 +102,9 @@ struct PolicyF64 {
     // leave one block per CU
     static constexpr bool kFuseYZ = false;
     static constexpr int kYZWaves = 1;
+    // waves per SIMD the z pass is held to (<= 168 VGPRs): its stage-2 form
+    // otherwise takes 171 and runs 2 waves
+    static constexpr int kZWaves = 3;
     template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *, TC U, double rho) {
         return upwind_flux<O, double>(win, U, rho);
@@ -150,6 +153,7 @@ struct PolicyF32 {
     static constexpr bool kBase = true;
     static constexpr bool kFuseYZ = true;  // y and z passes fused (WENO5)
     static constexpr int kYZWaves = 3;     // waves per SIMD of the fused pass (its LDS allows 3 blocks per CU)
+    static constexpr int kZWaves = 1;      // the z pass's registers as the compiler allocates them
     template <int O, bool BV>
     static __device__ __forceinline__ SFlux scal_flux(const TC *win, const double *bwin, TC U, double rho) {
         constexpr int R = Interp<O, float>::R;
@@ -269,7 +273,7 @@ __device__ __forceinline__ void rk_store(const Fields &fl, int v, unsigned o, un
 // rings as in the y pass below; the scalars' FP64 base window, which is the
 // same for every lane and lives in scalar registers, shifts.
 template <class P, int SO, int MO, int NSC, bool MOM, int RKM>
-__global__ __launch_bounds__(256) void adv_z(Geo g, Prof pr, Fields fl, int kchunk) {
+__global__ __launch_bounds__(256, P::kZWaves) void adv_z(Geo g, Prof pr, Fields fl, int kchunk) {
     using TS = typename P::TS;
     using TC = typename P::TC;
     using SFlux = typename P::SFlux;

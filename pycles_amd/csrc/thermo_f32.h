@@ -72,18 +72,11 @@ __device__ __forceinline__ float g_red(float T, float qt, float pd, float pv, fl
 
 // FP64 arithmetic of the residual check, without the library sequences'
 // handling of special cases (every argument here is a positive normal
-// number).  div64: the hardware reciprocal, two Newton steps and one
-// correction of the quotient.  log64: x = m 2^e with m in [sqrt(1/2),
+// number).  div64 (common.h): the hardware reciprocal, two Newton steps and
+// one correction of the quotient.  log64: x = m 2^e with m in [sqrt(1/2),
 // sqrt(2)), ln x = e ln 2 + 2 atanh(t), t = (m - 1)/(m + 1), the series to
 // t^17; within 1e-15 absolute of the C library's log on [6e-6, 20].  The
 // check resolves 3e-3 J/kg/K on terms of O(100) J/kg/K.
-__device__ __forceinline__ double div64(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
-    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
-    const double q = a * r;
-    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
-}
 __device__ __forceinline__ double log64(double x) {
     double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
     int e = __builtin_amdgcn_frexp_exp(x);

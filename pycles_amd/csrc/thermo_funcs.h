@@ -33,45 +33,49 @@ __device__ __forceinline__ double latent_heat_variable(double T) {
     return (2500.8 - 2.36 * TC + 0.0016 * TC * TC - 0.00006 * TC * TC * TC) * 1000.0;
 }
 
+// The divisions are div64 (common.h): the IEEE quotient for these operands
+// (pressures, temperatures, 1 - qt and sums of positive terms as divisors),
+// in fewer instructions; the secant step's f_2 - f_1 keeps the IEEE division.
+
 // thermodynamic_functions.h
-__device__ __forceinline__ double exner_c(double p0) { return pow((p0 / PC_P_TILDE), PC_KAPPA); }
+__device__ __forceinline__ double exner_c(double p0) { return pow(div64(p0, PC_P_TILDE), PC_KAPPA); }
 __device__ __forceinline__ double pd_c(double p0, double qt, double qv) {
-    return p0 * (1.0 - qt) / (1.0 - qt + PC_EPS_VI * qv);
+    return div64(p0 * (1.0 - qt), 1.0 - qt + PC_EPS_VI * qv);
 }
 __device__ __forceinline__ double pv_c(double p0, double qt, double qv) {
-    return p0 * PC_EPS_VI * qv / (1.0 - qt + PC_EPS_VI * qv);
+    return div64(p0 * PC_EPS_VI * qv, 1.0 - qt + PC_EPS_VI * qv);
 }
 __device__ __forceinline__ double cpm_c(double qt) { return (1.0 - qt) * PC_CPD + qt * PC_CPV; }
 __device__ __forceinline__ double qv_star_c(double p0, double qt, double pv) {
-    return PC_EPS_V * (1.0 - qt) * pv / (p0 - pv);
+    return div64(PC_EPS_V * (1.0 - qt) * pv, p0 - pv);
 }
 __device__ __forceinline__ double alpha_c(double p0, double T, double qt, double qv) {
-    return (PC_RD * T) / p0 * (1.0 - qt + PC_EPS_VI * qv);
+    return div64(PC_RD * T, p0) * (1.0 - qt + PC_EPS_VI * qv);
 }
 __device__ __forceinline__ double buoyancy_c(double alpha0, double alpha) {
-    return PC_G * (alpha - alpha0) / alpha0;
+    return div64(PC_G * (alpha - alpha0), alpha0);
 }
 __device__ __forceinline__ double theta_rho_c(double p0, double T, double qt, double qv) {
-    return T * (1.0 - qt + PC_EPS_VI * qv) / exner_c(p0);
+    return div64(T * (1.0 - qt + PC_EPS_VI * qv), exner_c(p0));
 }
 __device__ __forceinline__ double thetali_c(double p0, double T, double qt, double ql, double qi, double L) {
-    return T / exner_c(p0) * exp(-L * (ql / (1.0 - qt) + qi / (1.0 - qt)) / (T * PC_CPD));
+    return div64(T, exner_c(p0)) * exp(div64(-L * (div64(ql, 1.0 - qt) + div64(qi, 1.0 - qt)), T * PC_CPD));
 }
 
 // entropies.h:5-15
 __device__ __forceinline__ double sd_c(double pd, double T) {
-    return PC_SD_TILDE + PC_CPD * log(T / PC_T_TILDE) - PC_RD * log(pd / PC_P_TILDE);
+    return PC_SD_TILDE + PC_CPD * log(div64(T, PC_T_TILDE)) - PC_RD * log(div64(pd, PC_P_TILDE));
 }
 __device__ __forceinline__ double sv_c(double pv, double T) {
-    return PC_SV_TILDE + PC_CPV * log(T / PC_T_TILDE) - PC_RV * log(pv / PC_P_TILDE);
+    return PC_SV_TILDE + PC_CPV * log(div64(T, PC_T_TILDE)) - PC_RV * log(div64(pv, PC_P_TILDE));
 }
-__device__ __forceinline__ double sc_c(double L, double T) { return -L / T; }
+__device__ __forceinline__ double sc_c(double L, double T) { return div64(-L, T); }
 
 // thermodynamics_sa.h:10-15
 __device__ __forceinline__ double temperature_no_ql(double pd, double pv, double s, double qt) {
-    return PC_T_TILDE * exp((s - (1.0 - qt) * (PC_SD_TILDE - PC_RD * log(pd / PC_P_TILDE)) -
-                             qt * (PC_SV_TILDE - PC_RV * log(pv / PC_P_TILDE))) /
-                            ((1.0 - qt) * PC_CPD + qt * PC_CPV));
+    return PC_T_TILDE * exp(div64(s - (1.0 - qt) * (PC_SD_TILDE - PC_RD * log(div64(pd, PC_P_TILDE))) -
+                                      qt * (PC_SV_TILDE - PC_RV * log(div64(pv, PC_P_TILDE))),
+                                  (1.0 - qt) * PC_CPD + qt * PC_CPV));
 }
 
 // eos_c, thermodynamics_sa.h:17-66, lambda == 1; capped at max_iter secant
@@ -96,7 +100,7 @@ __device__ __forceinline__ int eos_sa(const Table &lt, double p0, double s, doub
     const double L_1 = latent_heat_variable(T_1);
     const double s_1 = sd_c(pd_1, T_1) * (1.0 - qt) + sv_c(pv_1, T_1) * qt + sc_c(L_1, T_1) * sigma_1;
     double f_1 = s - s_1;
-    double T_2 = T_1 + sigma_1 * L_1 / ((1.0 - qt) * PC_CPD + qv_star_1 * PC_CPV);
+    double T_2 = T_1 + div64(sigma_1 * L_1, (1.0 - qt) * PC_CPD + qv_star_1 * PC_CPV);
     double delta_T, qv_star_2, sigma_2;
     int it = 0;
     do {
@@ -131,19 +135,20 @@ __device__ __forceinline__ double bvf_sa(const Table &lt, double p0, const doubl
         const double Tk = T[1];
         const double Lv = latent_heat_variable(Tk);
         const double pv_star = lookup(lt, Tk);
-        const double rsl = PC_EPS_V * pv_star / (p0 - pv_star);
-        const double gamma_w = PC_G / PC_CPD * (1.0 / (1.0 - qt[1])) * (1.0 + Lv * rsl / (PC_RD * Tk)) /
-                               (cpm_c(qt[1]) / PC_CPD + Lv * Lv * (PC_EPS_V + rsl) * rsl / (PC_CPD * PC_RD * Tk * Tk));
+        const double rsl = div64(PC_EPS_V * pv_star, p0 - pv_star);
+        const double gamma_w = div64(PC_G / PC_CPD * div64(1.0, 1.0 - qt[1]) * (1.0 + div64(Lv * rsl, PC_RD * Tk)),
+                                     div64(cpm_c(qt[1]), PC_CPD) +
+                                         div64(Lv * Lv * (PC_EPS_V + rsl) * rsl, PC_CPD * PC_RD * Tk * Tk));
         const double dTdz = (0.5 * (T[1] + T[2]) - 0.5 * (T[0] + T[1])) * dzi;
         const double dqtdz = (0.5 * (qt[1] + qt[2]) - 0.5 * (qt[0] + qt[1])) * dzi;
-        return PC_G / Tk * (dTdz + gamma_w) * (1.0 + Lv * rsl / (PC_RD * Tk)) - dqtdz / (1.0 - qt[1]);
+        return div64(PC_G, Tk) * (dTdz + gamma_w) * (1.0 + div64(Lv * rsl, PC_RD * Tk)) - div64(dqtdz, 1.0 - qt[1]);
     }
-    return PC_G / thr[1] * (0.5 * (thr[1] + thr[2]) - 0.5 * (thr[0] + thr[1])) * dzi;
+    return div64(PC_G, thr[1]) * (0.5 * (thr[1] + thr[2]) - 0.5 * (thr[0] + thr[1])) * dzi;
 }
 
 // thermodynamics_dry.h:8-10
 __device__ __forceinline__ double eos_dry(double pd, double s) {
-    return PC_T_TILDE * (exp((s - PC_SD_TILDE + PC_RD * log(pd / PC_P_TILDE)) / PC_CPD));
+    return PC_T_TILDE * (exp(div64(s - PC_SD_TILDE + PC_RD * log(div64(pd, PC_P_TILDE)), PC_CPD)));
 }
 
 }  // namespace thermo
