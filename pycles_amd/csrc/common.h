@@ -181,10 +181,8 @@ template <typename T> struct Num;
 // the same result.
 __device__ __forceinline__ double div64(double a, double b) {
     double y = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-b, y, 1.0);
-    y = __builtin_fma(y, e, y);
+    y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
+    y = __builtin_fma(__builtin_fma(-b, y, 1.0), y, y);
     const double q = a * y;
     return __builtin_fma(__builtin_fma(-b, q, a), y, q);
 }

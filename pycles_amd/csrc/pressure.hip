@@ -225,12 +225,14 @@ __global__ __launch_bounds__(512) void div_fftx_kernel(Geo g, const TS *u, const
             const int rb = t >> logx;
             TS d;
             if (sizeof(TS) == 8) {
+                // the profiles are positive normal numbers: div64 is the
+                // IEEE quotient (common.h)
                 double dv = 0.0;
-                dv += ((double)uc[q] / ah - (double)um[q] / ah) * dxi0;
-                dv += ((double)vc[q] / ah - (double)vm[q] / ah) * dxi1;
+                dv += (div64((double)uc[q], ah) - div64((double)um[q], ah)) * dxi0;
+                dv += (div64((double)vc[q], ah) - div64((double)vm[q], ah)) * dxi1;
                 const double wk = (double)wc[q] - mk;
                 const double wkm = (double)wm[q] - mkm;
-                dv += ((wk) / a0 - (wkm) / a0m) * dxi2;
+                dv += (div64(wk, a0) - div64(wkm, a0m)) * dxi2;
                 d = (TS)dv;
             } else {
                 // FP32 path: rho0*u in FP32 except the level means and profiles

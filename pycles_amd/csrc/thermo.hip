@@ -85,15 +85,15 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
                 }
             }
             double T, qv, ql, qi;
-            const int it = thermo::eos_sa(lt, p0, sk, qt, &T, &qv, &ql, &qi, a.max_iter, a.sa_tol);
+            const int it = thermo::eos_sa<thermo::DivFast>(lt, p0, sk, qt, &T, &qv, &ql, &qi, a.max_iter, a.sa_tol);
             if (it > 0 && own) {
                 ctr[0] += 1;
                 ctr[1] += it;
                 if (a.max_iter > 0 && it >= a.max_iter) ctr[2] += 1;
             }
-            const double al = thermo::alpha_c(p0, T, qt, qv);
+            const double al = thermo::alpha_c<thermo::DivFast>(p0, T, qt, qv);
             double b = 0.0;
-            if (k < g.nzg - 1) b = thermo::buoyancy_c(a0l[l], al);
+            if (k < g.nzg - 1) b = thermo::buoyancy_c<thermo::DivFast>(a0l[l], al);
             const double ex = exner[l];
             double thr = 0.0;
             if (k >= 1 && k < g.nzg - 1) thr = T * (1.0 - qt + PC_EPS_VI * qv) / ex;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void thermo_sa_f64_kernel(Geo g, SAArgs a) {
             thw[0] = thw[1]; thw[1] = thw[2]; thw[2] = thr;
             const int m = k - 1;
             if (bvfo && own_m && m >= 2 && m < g.nzg - 2)
-                bvfo[o - g.plane] = thermo::bvf_sa(lt, p0l[l - 1], Tw, qtw, qvw, thw, dzi);
+                bvfo[o - g.plane] = thermo::bvf_sa<thermo::DivFast>(lt, p0l[l - 1], Tw, qtw, qvw, thw, dzi);
         }
     }
     flush_counters(cnt, ctr, a.counters, tid);
@@ -240,10 +240,10 @@ __global__ __launch_bounds__(256) void thermo_dry_kernel(Geo g, DryArgs a) {
         const double p0 = a.p0[k];
         double sv = (double)s[o];
         if (a.sbase) sv += a.sbase[k];
-        const double T = thermo::eos_dry(p0, sv);
-        const double al = thermo::alpha_c(p0, T, 0.0, 0.0);
+        const double T = thermo::eos_dry<thermo::DivFast>(p0, sv);
+        const double al = thermo::alpha_c<thermo::DivFast>(p0, T, 0.0, 0.0);
         double b = 0.0;
-        if (k < g.nzg - 1) b = thermo::buoyancy_c(a.alpha0_half[k], al);
+        if (k < g.nzg - 1) b = thermo::buoyancy_c<thermo::DivFast>(a.alpha0_half[k], al);
         double th = 0.0;
         if (k >= 1) th = T / thermo::exner_c(p0);
         if (own) {
